@@ -1,0 +1,192 @@
+"""Benchmark of the DE-MCMC hot path on one MI355X.
+
+Workload (default c3): hierarchical B x v LBA, 32 subjects x 768 trials,
+13 subject parameters (26 population parameters), 39 chains, synthetic data
+from the LBA generation rule. A step is one full DE-MCMC iteration (population
+step + every subject's crossover / migration over all its trials).
+
+Measured on the same inputs, all resident in HBM before the timed region:
+  value          iterations/s of the batched path (FP32 transcendentals, FP64
+                 sums and sampler state, hipGraph-replayed launches)
+  plain_fp64     iterations/s of the plain FP64 GPU path (the reference's
+                 arithmetic and sequential chain order, one launch per chain step)
+  max_rel_dLL    max |LL_batched / LL_fp64 - 1| over every stored subject state
+  roofline       of the subject sweep kernel (dominant), VALU issue bound
+  cpu_baseline   the FP64 oracle restatement of the reference sampler, one core
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("DE-MCMC iter/s + trial-LL evals/s: batched FP32 vs plain FP64 GPU, "
+          "|ΔLL|≤1e-6")
+
+# VALU issue slots per trial-likelihood evaluation of the batched kernel
+# (one slot = one lane for one VALU cycle; quarter-rate transcendentals count
+# 4), measured with rocprofv3 SQ_INSTS_VALU / SQ_INSTS_VALU_TRANS_F32 on the
+# subject sweep kernel; see profiles/.
+W_SLOTS = {2: 400.0, 4: 800.0}
+# 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
+VALU_PEAK_TSLOTS = 256 * 4 * 16 * 2.4e9 / 1e12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--workload", default="c3")
+    ap.add_argument("--plain-steps", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--profile-iters", type=int, default=20)
+    return ap.parse_args()
+
+
+def cpu_baseline(w, budget_s):
+    """Oracle (FP64, single thread) on the same workload for a bounded number
+    of iterations; returns iterations/s and the sample description."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from ggdmc_amd import _lib
+    j = w.jobs[0]
+    om = O.Model(w.model.winner, w.model.par_src, w.model.par_const, w.model.npar)
+
+    def opr(p):
+        return O.Prior(p.dist, p.p0, p.p1, p.lower, p.upper)
+
+    def one(n_iter):
+        cfg = O.config(nmc=n_iter + 1, nchain=w.nchain, nparameter_de=j.nparameter_de,
+                       sub_migration_prob=w.sub_migration_prob,
+                       pop_migration_prob=w.pop_migration_prob, seed=j.seed)
+        t = time.perf_counter()
+        if j.kind == _lib.KIND_HIER:
+            datas = j.subjects
+            rt = np.concatenate([d.rt for d in datas])
+            so = np.concatenate([[0], np.cumsum([len(d.rt) for d in datas])])
+            co = np.stack([d.cell_off for d in datas])
+            O.run_hier(cfg, om, opr(j.p_prior), opr(j.h_prior), rt, so, co, *j.phi_start,
+                       np.stack([s[0] for s in j.subject_starts]),
+                       np.stack([s[1] for s in j.subject_starts]),
+                       np.stack([s[2] for s in j.subject_starts]))
+        else:
+            O.run_subject(cfg, om, opr(j.p_prior), j.subject.rt, j.subject.cell_off, *j.start)
+        return time.perf_counter() - t
+
+    t1 = one(1)
+    n = max(2, int(budget_s / max(t1, 1e-6)))
+    dt = one(n)
+    return n / dt, n, dt
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build runs no multi-GPU configuration "
+              "(--gpus %d requested)" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    import ggdmc_amd as g
+    from ggdmc_amd import workloads
+    if g.device_count() < 1:
+        print("no gfx950 GPU visible", file=sys.stderr)
+        sys.exit(1)
+
+    t_build = time.time()
+    w = workloads.make(a.workload)
+    n_acc = w.model.n_acc
+    total = a.warmup + a.steps + a.profile_iters
+    thin = max(1, total // 1000)
+    nmc = total // thin + 2
+    cfg = w.config(nmc=nmc, thin=thin)
+    sess = g.Session(cfg, w.model, w.jobs)
+    tpt, ntiles = sess.tile()
+    print("[bench] workload %s: %s; %d subjects, %d trials; tile %d trials, %d tiles; "
+          "setup %.1fs" % (w.name, w.description, w.n_subjects, w.n_trials, tpt, ntiles,
+                           time.time() - t_build), file=sys.stderr, flush=True)
+
+    sess.advance(a.warmup)
+    e0 = sess.trial_evals()
+    t0 = time.perf_counter()
+    ms = sess.advance(a.steps)
+    wall = time.perf_counter() - t0
+    e1 = sess.trial_evals()
+    iters_per_s = a.steps / (ms / 1e3)
+    evals_per_s = (e1 - e0) / (ms / 1e3)
+    print("[bench] batched: %d iterations in %.1f ms (wall %.1f ms): %.0f iter/s, %.3g trial-evals/s"
+          % (a.steps, ms, wall * 1e3, iters_per_s, evals_per_s), file=sys.stderr, flush=True)
+
+    # dominant kernel, timed launch by launch with HIP events on its stream
+    e2 = sess.trial_evals()
+    k_ms, n_launch = sess.profile(a.profile_iters)
+    e3 = sess.trial_evals()
+    evals_per_launch = (e3 - e2) / max(n_launch, 1)
+    achieved = evals_per_launch * W_SLOTS[n_acc] / (k_ms / 1e3) / 1e12
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TSLOTS, 2),
+                "unit": "Tslot/s", "frac": round(achieved / VALU_PEAK_TSLOTS, 4),
+                "traffic": None, "kernel": "subject_sweep_kernel",
+                "kernel_ms": round(k_ms, 4), "evals_per_launch": evals_per_launch,
+                "slots_per_eval": W_SLOTS[n_acc]}
+    res = sess.download()
+    sess.close()
+
+    # per-proposal |dLL|: every stored subject state against the FP64 path
+    worst = 0.0
+    n_cmp = 0
+    for job, r in zip(w.jobs, res):
+        datas = [job.subject] if job.kind == g.KIND_SUBJECT else job.subjects
+        for d, post in zip(datas, r["subjects"]):
+            k = post.nmc - 1
+            th = post.theta[:, :, k].T
+            ref = g.loglike(w.model, d, th, g.FP64_PLAIN)
+            worst = max(worst, float(np.max(np.abs(post.log_likelihoods[:, k] / ref - 1))))
+            n_cmp += len(ref)
+
+    plain = None
+    if not a.no_plain:
+        pcfg = w.config(nmc=a.plain_steps + 3, precision=g.FP64_PLAIN)
+        ps = g.Session(pcfg, w.model, w.jobs)
+        ps.advance(2)
+        pms = ps.advance(a.plain_steps)
+        ps.close()
+        plain = a.plain_steps / (pms / 1e3)
+        print("[bench] plain FP64: %d iterations in %.1f ms: %.1f iter/s"
+              % (a.plain_steps, pms, plain), file=sys.stderr, flush=True)
+
+    cpu = None
+    if not a.no_cpu:
+        cps, n_cpu, dt = cpu_baseline(w, a.cpu_seconds)
+        cpu = {"value": round(cps, 4), "unit": "iter/s", "cores": 1, "kind": "port",
+               "sample": "%d iterations of the same %s workload in %.1f s (FP64 oracle, "
+                         "one thread)" % (n_cpu, w.name, dt)}
+        print("[bench] cpu oracle: %s" % cpu["sample"], file=sys.stderr, flush=True)
+
+    out = {
+        "metric": METRIC, "value": round(iters_per_s, 2), "unit": "iter/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms / a.steps, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 transcendentals + fp64 sums/state", "data": "synthetic",
+        "config": {"workload": "%s: %s" % (w.name, w.description), "subjects": w.n_subjects,
+                   "trials": w.n_trials, "chains": w.nchain, "sweep_groups": 2,
+                   "trials_per_tile": tpt, "tiles": ntiles},
+        "trial_evals_per_s": round(evals_per_s, 1),
+        "plain_fp64_iter_per_s": round(plain, 3) if plain else None,
+        "speedup_vs_plain_fp64": round(iters_per_s / plain, 2) if plain else None,
+        "speedup_vs_cpu": round(iters_per_s / cpu["value"], 1) if cpu else None,
+        "max_rel_dLL": worst, "dLL_states_checked": n_cmp,
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,892 @@
+/*
+ * Host engine and C ABI of libggdmc_hip (see include/ggdmc_hip.h).
+ *
+ * A session uploads a batch of runs once (trials, designs, priors, start
+ * states), keeps every state in HBM, and advances all runs together:
+ *   batched path : per iteration one population launch (hierarchical /
+ *                  hyper runs) and one subject launch per sweep (all chain
+ *                  groups in-kernel when every subject is one tile) or per
+ *                  (sweep, group); blocks of iterations are captured in a
+ *                  hipGraph and replayed.
+ *   plain path   : the reference's sequential order, one launch per chain
+ *                  step, no graph (the FP64 comparator).
+ * The reference's own driver loop is de_class::run_chains / run_hchains
+ * (src/de.cpp:201-242, :272-383); the Rcpp glue it replaces is
+ * src/de2R.cpp:8-171.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/ggdmc_hip.h"
+#include "engine.h"
+
+using namespace ggdmc;
+
+extern "C" {
+size_t ggdmc_internal_sweep_lds(int n, int P, int ng, int n_cell, int n_acc, int chain_batch);
+size_t ggdmc_internal_phi_lds(int n, int P);
+hipError_t ggdmc_internal_launch_sweep(const Dev *d, const SweepArgs *a, int mode, int tpl,
+                                       size_t lds, hipStream_t stream);
+hipError_t ggdmc_internal_launch_phi(const Dev *d, uint32_t iter_off, size_t lds, hipStream_t stream);
+hipError_t ggdmc_internal_launch_plain_subject(const Dev *d, uint32_t iter, int sweep, int j,
+                                               hipStream_t stream);
+hipError_t ggdmc_internal_launch_plain_phi(const Dev *d, uint32_t iter, int sweep, int j,
+                                           const double *rows, const int64_t *rows_off,
+                                           hipStream_t stream);
+hipError_t ggdmc_internal_launch_store(const Dev *d, uint32_t iter, hipStream_t stream);
+hipError_t ggdmc_internal_launch_hyper_stats(const Dev *d, const double *rows, int S, int f,
+                                             hipStream_t stream);
+hipError_t ggdmc_internal_launch_loglike(const Dev *d, const double *thetas, int nq, int s,
+                                         int mode, size_t lds, hipStream_t stream, double *out);
+}
+
+namespace {
+
+thread_local std::string g_error;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess)
+        throw Error(e == hipErrorOutOfMemory ? GGDMC_ENOMEM : GGDMC_EHIP,
+                    std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPCHECK(x) hip_check((x), #x)
+
+void require(bool ok, const std::string &msg) {
+    if (!ok) throw Error(GGDMC_EINVAL, msg);
+}
+
+__global__ void set_base_kernel(uint32_t *p, uint32_t v) { *p = v; }
+
+/* Device memory owned by a session; freed with it. */
+struct Pool {
+    std::vector<void *> ptrs;
+    template <class T>
+    T *alloc(size_t n) {
+        void *p = nullptr;
+        if (n == 0) n = 1;
+        HIPCHECK(hipMalloc(&p, n * sizeof(T)));
+        ptrs.push_back(p);
+        return static_cast<T *>(p);
+    }
+    template <class T>
+    T *upload(const std::vector<T> &v, hipStream_t st) {
+        T *p = alloc<T>(v.size());
+        if (!v.empty()) HIPCHECK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+        return p;
+    }
+    ~Pool() {
+        for (void *p : ptrs) (void)hipFree(p);
+    }
+};
+
+int device_lds_limit() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 65536;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 65536;
+    return (int)std::min<size_t>(prop.sharedMemPerBlock, 65536);
+}
+
+int device_cus() {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    return cus;
+}
+
+} // namespace
+
+/* ================================================================ session */
+struct ggdmc_session {
+    ggdmc_config cfg{};
+    std::vector<ggdmc_job> jobs;
+    Pool pool;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    Dev d{};
+    int mode = GGDMC_MIXED_BATCHED;
+    int tpl = 2;
+    int tpt = 0;                 /* trials per tile; 0 = whole subject */
+    int chain_batch = 0;
+    size_t sweep_lds = 0, phi_lds = 0;
+    int nsweep_sub = 1;
+    bool fused_groups = true;
+    uint32_t next_iter = 1;      /* next local iteration to run (1-based) */
+    uint32_t last_iter = 0;      /* nmc*thin - 1 */
+    uint32_t *iter_base = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int graph_iters = 0;
+    float last_ms = 0.f;
+    double *hyper_rows = nullptr;
+    int64_t *hyper_off = nullptr;
+    std::vector<int64_t> subj_ntr;   /* host copy of trial counts */
+    std::vector<int> subj_of_job;    /* first subject of each job */
+    std::vector<int> phi_of_job;
+    int64_t max_ntr = 0;
+    std::vector<double> hyper_rows_h;
+
+    ~ggdmc_session() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+    }
+
+    void build(const ggdmc_config *c, const ggdmc_model *model, const ggdmc_job *js, int nj);
+    void set_tiles(int tpt_new);
+    void enqueue_iteration(uint32_t off, bool graph_capture);
+    void run_batched(uint32_t n_iter);
+    void run_plain(uint32_t n_iter);
+    void autotune();
+    void capture(int K);
+    void drop_graph() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+        graph_iters = 0;
+    }
+};
+
+static void check_prior(const ggdmc_prior *p, int npar, const char *what) {
+    require(p != nullptr, std::string(what) + " is null");
+    require(p->npar == npar, std::string(what) + ": npar mismatch");
+    require(p->dist && p->p0 && p->p1 && p->lower && p->upper, std::string(what) + ": null array");
+    for (int k = 0; k < npar; ++k)
+        require(p->dist[k] == GGDMC_DIST_UNIF || p->dist[k] == GGDMC_DIST_TNORM,
+                std::string(what) + ": only dist_id 1 (tnorm) and 6 (unif) are implemented");
+}
+
+void ggdmc_session::build(const ggdmc_config *c, const ggdmc_model *model, const ggdmc_job *js,
+                          int nj) {
+    require(c != nullptr && js != nullptr && nj > 0, "config / jobs missing");
+    cfg = *c;
+    jobs.assign(js, js + nj);
+    const int n = cfg.nchain;
+    require(n > 2, "Require three or more chains.");
+    require(n <= kMaxChains, "at most 256 chains per run");
+    require(cfg.nmc >= 1 && cfg.thin >= 1, "nmc and thin must be >= 1");
+    mode = cfg.precision;
+    require(mode == GGDMC_FP64_PLAIN || mode == GGDMC_MIXED_BATCHED || mode == GGDMC_FP64_BATCHED,
+            "unknown precision");
+    int G = cfg.sweep_groups;
+    if (G == 0) G = mode == GGDMC_FP64_PLAIN ? n : 2;
+    require(G >= 2 && G <= n, "sweep_groups must be in [2, nchain]");
+    require(mode != GGDMC_FP64_PLAIN || G == n, "the plain path runs the reference order (sweep_groups = nchain)");
+    require(n - (n + G - 1) / G >= 2, "each chain group must leave two partners");
+
+    bool any_lba = false;
+    int P = -1;
+    for (const auto &j : jobs) {
+        require(j.kind >= GGDMC_KIND_SUBJECT && j.kind <= GGDMC_KIND_HIER, "unknown job kind");
+        if (j.kind != GGDMC_KIND_HYPER) any_lba = true;
+    }
+    if (any_lba) {
+        require(model != nullptr && model->winner && model->par_src && model->par_const, "model missing");
+        require(model->n_acc >= 1 && model->n_cell >= 1 && model->npar >= 1, "empty model");
+        require(model->n_cell <= 256, "at most 256 cells");
+        require(model->n_cell * model->n_acc <= 256 && model->npar <= 64, "design too large");
+        P = model->npar;
+        for (int e = 0; e < model->n_cell * model->n_acc; ++e) {
+            for (int f = 0; f < 6; ++f) {
+                const int src = model->par_src[e * 6 + f];
+                require(src >= -1 && src < P, "par_src out of range");
+                if (f == 4) require(src == -1 && model->par_const[e * 6 + 4] == 0.0,
+                                    "st0 must be the constant 0");
+            }
+        }
+        for (int cidx = 0; cidx < model->n_cell; ++cidx)
+            require(model->winner[cidx] >= 0 && model->winner[cidx] < model->n_acc, "winner out of range");
+    }
+    for (const auto &j : jobs) {
+        if (j.kind == GGDMC_KIND_HYPER) {
+            require(j.p_prior != nullptr, "hyper run needs p_prior");
+            if (P < 0) P = j.p_prior->npar;
+            require(j.p_prior->npar == P, "all runs must share the parameter count");
+        }
+    }
+    require(P >= 1 && 2 * P <= 128, "parameter count out of range");
+
+    HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHECK(hipEventCreate(&ev0));
+    HIPCHECK(hipEventCreate(&ev1));
+
+    /* ---- host-side flattening */
+    std::vector<double> rt;
+    std::vector<uint16_t> tcell;
+    std::vector<int64_t> subj_off{0};
+    std::vector<int32_t> subj_job, subj_unit;
+    std::vector<JobDev> jd;
+    std::vector<int32_t> phi_job;
+    std::vector<int32_t> pr_dist;
+    std::vector<double> pr_p0, pr_p1, pr_lo, pr_up;
+    std::vector<double> th0, lp0, ll0, ph0, plp0, pll0;
+    std::vector<int64_t> hyper_off_h;
+    auto add_prior = [&](const ggdmc_prior *p) {
+        const int off = (int)pr_dist.size();
+        for (int k = 0; k < p->npar; ++k) {
+            pr_dist.push_back(p->dist[k]);
+            pr_p0.push_back(p->p0[k]);
+            pr_p1.push_back(p->p1[k]);
+            pr_lo.push_back(p->lower[k]);
+            pr_up.push_back(p->upper[k]);
+        }
+        return off;
+    };
+    auto add_state = [&](const ggdmc_state *st, int np, std::vector<double> &th,
+                         std::vector<double> &lp, std::vector<double> &ll) {
+        require(st && st->theta && st->lp && st->ll, "start state missing");
+        th.insert(th.end(), st->theta, st->theta + (size_t)np * n);
+        lp.insert(lp.end(), st->lp, st->lp + n);
+        ll.insert(ll.end(), st->ll, st->ll + n);
+    };
+    auto add_subject = [&](const ggdmc_subject *sb, int job, int unit) {
+        require(sb && sb->cell_off && (sb->rt || sb->ntrial == 0), "subject data missing");
+        require(sb->cell_off[0] == 0 && sb->cell_off[model->n_cell] == sb->ntrial,
+                "cell_off must start at 0 and end at ntrial");
+        for (int cidx = 0; cidx < model->n_cell; ++cidx) {
+            require(sb->cell_off[cidx + 1] >= sb->cell_off[cidx], "cell_off must be non-decreasing");
+            for (int64_t i = sb->cell_off[cidx]; i < sb->cell_off[cidx + 1]; ++i) {
+                rt.push_back(sb->rt[i]);
+                tcell.push_back((uint16_t)cidx);
+            }
+        }
+        subj_off.push_back((int64_t)rt.size());
+        subj_job.push_back(job);
+        subj_unit.push_back(unit);
+        subj_ntr.push_back(sb->ntrial);
+        max_ntr = std::max<int64_t>(max_ntr, sb->ntrial);
+    };
+
+    for (int ji = 0; ji < nj; ++ji) {
+        const ggdmc_job &j = jobs[ji];
+        JobDev J{};
+        J.kind = j.kind;
+        J.k0 = (uint32_t)j.seed;
+        J.k1 = (uint32_t)(j.seed >> 32);
+        const int npde = j.nparameter_de > 0 ? j.nparameter_de : (j.kind == GGDMC_KIND_SUBJECT ? P : 2 * P);
+        J.gamma = cfg.gamma_precursor / std::sqrt(2.0 * npde);
+        J.first_subject = (int)subj_job.size();
+        J.phi_index = -1;
+        subj_of_job.push_back(J.first_subject);
+        if (j.kind == GGDMC_KIND_SUBJECT) {
+            check_prior(j.p_prior, P, "p_prior");
+            require(j.subjects && j.subject_starts && j.subject_outs, "subject run needs data, start, out");
+            J.prior_off = add_prior(j.p_prior);
+            J.hprior_off = 0;
+            J.nsubject = 1;
+            add_subject(&j.subjects[0], ji, 0);
+            add_state(&j.subject_starts[0], P, th0, lp0, ll0);
+        } else {
+            check_prior(j.p_prior, P, "p_prior");
+            check_prior(j.h_prior, 2 * P, "h_prior");
+            for (int k = 0; k < P; ++k)
+                require(j.p_prior->dist[k] == GGDMC_DIST_TNORM, "population likelihood needs tnorm p_prior");
+            require(j.nsubject >= 1, "nsubject must be >= 1");
+            require(j.phi_start && j.phi_out, "population start / out missing");
+            J.prior_off = add_prior(j.p_prior);
+            J.hprior_off = add_prior(j.h_prior);
+            J.nsubject = j.nsubject;
+            J.phi_index = (int)phi_job.size();
+            phi_job.push_back(ji);
+            add_state(j.phi_start, 2 * P, ph0, plp0, pll0);
+            if (j.kind == GGDMC_KIND_HIER) {
+                require(j.subjects && j.subject_starts && j.subject_outs, "hierarchical run needs subjects");
+                for (int s = 0; s < j.nsubject; ++s) {
+                    add_subject(&j.subjects[s], ji, s);
+                    add_state(&j.subject_starts[s], P, th0, lp0, ll0);
+                }
+                hyper_off_h.push_back(0);
+            } else {
+                require(j.hyper_data != nullptr, "hyper run needs hyper_data");
+                hyper_off_h.push_back((int64_t)hyper_rows_h.size());
+                hyper_rows_h.insert(hyper_rows_h.end(), j.hyper_data, j.hyper_data + (size_t)j.nsubject * P);
+            }
+        }
+        phi_of_job.push_back(J.phi_index);
+        jd.push_back(J);
+    }
+
+    const int S = (int)subj_job.size();
+    const int F = (int)phi_job.size();
+    d.P = P;
+    d.n_acc = any_lba ? model->n_acc : 1;
+    d.n_cell = any_lba ? model->n_cell : 1;
+    d.nchain = n;
+    d.G = G;
+    d.nmc = cfg.nmc;
+    d.thin = cfg.thin;
+    d.blocked_sub = cfg.is_pblocked != 0;
+    d.blocked_pop = cfg.is_hblocked != 0;
+    d.rp = cfg.rp;
+    d.sub_mig = cfg.sub_migration_prob;
+    d.pop_mig = cfg.pop_migration_prob;
+    d.S = S;
+    d.n_phi = F;
+    d.n_jobs = nj;
+    d.philox_offset = (uint32_t)cfg.iter_offset;
+
+    if (any_lba) {
+        d.par_src = pool.upload(std::vector<int32_t>(model->par_src, model->par_src + model->n_cell * model->n_acc * 6), stream);
+        d.par_const = pool.upload(std::vector<double>(model->par_const, model->par_const + model->n_cell * model->n_acc * 6), stream);
+        d.winner = pool.upload(std::vector<int32_t>(model->winner, model->winner + model->n_cell), stream);
+    }
+    d.rt = pool.upload(rt, stream);
+    d.tcell = pool.upload(tcell, stream);
+    d.subj_off = pool.upload(subj_off, stream);
+    d.subj_job = pool.upload(subj_job, stream);
+    d.subj_unit = pool.upload(subj_unit, stream);
+    d.jobs = pool.upload(jd, stream);
+    d.phi_job = pool.upload(phi_job, stream);
+    d.pr_dist = pool.upload(pr_dist, stream);
+    d.pr_p0 = pool.upload(pr_p0, stream);
+    d.pr_p1 = pool.upload(pr_p1, stream);
+    d.pr_lo = pool.upload(pr_lo, stream);
+    d.pr_up = pool.upload(pr_up, stream);
+    d.theta = pool.upload(th0, stream);
+    d.lp = pool.upload(lp0, stream);
+    d.ll = pool.upload(ll0, stream);
+    d.phi = pool.upload(ph0, stream);
+    d.phi_lp = pool.upload(plp0, stream);
+    d.phi_ll = pool.upload(pll0, stream);
+    d.stats = pool.alloc<double>((size_t)std::max(F, 1) * n * P * 4);
+    d.counter = pool.alloc<uint32_t>((size_t)S + F + 1);
+    HIPCHECK(hipMemsetAsync(d.counter, 0, sizeof(uint32_t) * (S + F + 1), stream));
+    d.evals = pool.alloc<unsigned long long>(1);
+    HIPCHECK(hipMemsetAsync(d.evals, 0, sizeof(unsigned long long), stream));
+    iter_base = pool.alloc<uint32_t>(1);
+    d.iter_base = iter_base;
+    hyper_rows = pool.upload(hyper_rows_h, stream);
+    hyper_off = pool.upload(hyper_off_h, stream);
+
+    /* outputs, column 0 = start state */
+    const size_t nmc = (size_t)cfg.nmc;
+    d.out_theta = pool.alloc<double>(std::max<size_t>((size_t)S * nmc * n * P, 1));
+    d.out_lp = pool.alloc<double>(std::max<size_t>((size_t)S * nmc * n, 1));
+    d.out_ll = pool.alloc<double>(std::max<size_t>((size_t)S * nmc * n, 1));
+    d.phi_out_theta = pool.alloc<double>(std::max<size_t>((size_t)F * nmc * n * 2 * P, 1));
+    d.phi_out_lp = pool.alloc<double>(std::max<size_t>((size_t)F * nmc * n, 1));
+    d.phi_out_ll = pool.alloc<double>(std::max<size_t>((size_t)F * nmc * n, 1));
+    for (int s = 0; s < S; ++s) {
+        HIPCHECK(hipMemcpyAsync(d.out_theta + (size_t)s * nmc * n * P, d.theta + (size_t)s * n * P,
+                                sizeof(double) * n * P, hipMemcpyDeviceToDevice, stream));
+        HIPCHECK(hipMemcpyAsync(d.out_lp + (size_t)s * nmc * n, d.lp + (size_t)s * n, sizeof(double) * n,
+                                hipMemcpyDeviceToDevice, stream));
+        HIPCHECK(hipMemcpyAsync(d.out_ll + (size_t)s * nmc * n, d.ll + (size_t)s * n, sizeof(double) * n,
+                                hipMemcpyDeviceToDevice, stream));
+    }
+    for (int f = 0; f < F; ++f) {
+        HIPCHECK(hipMemcpyAsync(d.phi_out_theta + (size_t)f * nmc * n * 2 * P, d.phi + (size_t)f * n * 2 * P,
+                                sizeof(double) * n * 2 * P, hipMemcpyDeviceToDevice, stream));
+        HIPCHECK(hipMemcpyAsync(d.phi_out_lp + (size_t)f * nmc * n, d.phi_lp + (size_t)f * n, sizeof(double) * n,
+                                hipMemcpyDeviceToDevice, stream));
+        HIPCHECK(hipMemcpyAsync(d.phi_out_ll + (size_t)f * nmc * n, d.phi_ll + (size_t)f * n, sizeof(double) * n,
+                                hipMemcpyDeviceToDevice, stream));
+    }
+    for (int f = 0; f < F; ++f) {
+        const ggdmc_job &j = jobs[phi_job[f]];
+        if (j.kind == GGDMC_KIND_HYPER)
+            HIPCHECK(ggdmc_internal_launch_hyper_stats(&d, hyper_rows + hyper_off_h[f], j.nsubject, f, stream));
+    }
+
+    nsweep_sub = cfg.is_pblocked ? P : 1;
+    last_iter = (uint32_t)(cfg.nmc * cfg.thin - 1);
+    next_iter = 1;
+
+    /* LDS budget: chains whose AccK tables are resident at once */
+    const int ng = (n + G - 1) / G + 1;
+    const int limit = device_lds_limit();
+    chain_batch = ng;
+    while (chain_batch > 1 &&
+           ggdmc_internal_sweep_lds(n, P, ng, d.n_cell, d.n_acc, chain_batch) > (size_t)std::min(limit, 48 * 1024))
+        --chain_batch;
+    sweep_lds = ggdmc_internal_sweep_lds(n, P, ng, d.n_cell, d.n_acc, chain_batch);
+    require(sweep_lds <= (size_t)limit, "design too large for LDS");
+    phi_lds = ggdmc_internal_phi_lds(n, P);
+    require(phi_lds <= (size_t)limit, "population state too large for LDS");
+
+    tpl = 2;
+    set_tiles(cfg.trials_per_tile > 0 ? cfg.trials_per_tile : 0);
+    HIPCHECK(hipStreamSynchronize(stream));
+}
+
+/* Splits every subject's trials into tiles of tpt_new trials (rounded up to a
+ * multiple of 64; 0 = one tile per subject). */
+void ggdmc_session::set_tiles(int tpt_new) {
+    if (tpt_new > 0) tpt_new = ((tpt_new + 63) / 64) * 64;
+    tpt = tpt_new;
+    std::vector<int32_t> tile_subj, tile0, ntile;
+    std::vector<int64_t> lo, hi;
+    int64_t off = 0;
+    for (int s = 0; s < d.S; ++s) {
+        const int64_t ntr = subj_ntr[s];
+        const int64_t nt = (tpt == 0 || ntr <= tpt) ? 1 : (ntr + tpt - 1) / tpt;
+        tile0.push_back((int32_t)tile_subj.size());
+        ntile.push_back((int32_t)nt);
+        for (int64_t t = 0; t < nt; ++t) {
+            tile_subj.push_back(s);
+            const int64_t a = off + (nt == 1 ? 0 : t * tpt);
+            const int64_t b = nt == 1 ? off + ntr : std::min(off + ntr, a + tpt);
+            lo.push_back(a);
+            hi.push_back(b);
+        }
+        off += ntr;
+    }
+    fused_groups = std::all_of(ntile.begin(), ntile.end(), [](int32_t v) { return v == 1; });
+    d.n_tiles = (int)tile_subj.size();
+    d.tile_subj = pool.upload(tile_subj, stream);
+    d.subj_tile0 = pool.upload(tile0, stream);
+    d.subj_ntile = pool.upload(ntile, stream);
+    d.tile_lo = pool.upload(lo, stream);
+    d.tile_hi = pool.upload(hi, stream);
+    d.partial = pool.alloc<double>((size_t)std::max(d.n_tiles, 1) * d.nchain);
+    drop_graph();
+}
+
+void ggdmc_session::enqueue_iteration(uint32_t off, bool) {
+    if (d.n_phi > 0) HIPCHECK(ggdmc_internal_launch_phi(&d, off, phi_lds, stream));
+    if (d.S == 0) return;
+    const int lk = mode == GGDMC_FP64_BATCHED ? GGDMC_FP64_BATCHED : GGDMC_MIXED_BATCHED;
+    for (int sw = 0; sw < nsweep_sub; ++sw) {
+        if (fused_groups) {
+            SweepArgs a{off, sw, 0, d.G, sw == nsweep_sub - 1, chain_batch};
+            HIPCHECK(ggdmc_internal_launch_sweep(&d, &a, lk, tpl, sweep_lds, stream));
+        } else {
+            for (int g = 0; g < d.G; ++g) {
+                SweepArgs a{off, sw, g, g + 1, (sw == nsweep_sub - 1 && g == d.G - 1), chain_batch};
+                HIPCHECK(ggdmc_internal_launch_sweep(&d, &a, lk, tpl, sweep_lds, stream));
+            }
+        }
+    }
+}
+
+void ggdmc_session::capture(int K) {
+    drop_graph();
+    if (K < 2) return;
+    hipGraph_t g = nullptr;
+    HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
+        for (int k = 0; k < K; ++k) enqueue_iteration((uint32_t)k, true);
+    } catch (...) {
+        (void)hipStreamEndCapture(stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    HIPCHECK(hipStreamEndCapture(stream, &g));
+    const hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHECK(e);
+    graph_iters = K;
+}
+
+void ggdmc_session::run_batched(uint32_t n_iter) {
+    while (n_iter > 0) {
+        if (graph && n_iter >= (uint32_t)graph_iters) {
+            set_base_kernel<<<1, 1, 0, stream>>>(iter_base, next_iter);
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipGraphLaunch(graph, stream));
+            next_iter += graph_iters;
+            n_iter -= graph_iters;
+        } else {
+            set_base_kernel<<<1, 1, 0, stream>>>(iter_base, next_iter);
+            HIPCHECK(hipGetLastError());
+            enqueue_iteration(0, false);
+            next_iter += 1;
+            n_iter -= 1;
+        }
+    }
+}
+
+void ggdmc_session::run_plain(uint32_t n_iter) {
+    int nsw_phi = 0;
+    for (int f = 0; f < d.n_phi; ++f) {
+        const ggdmc_job &j = jobs[phi_of_job.empty() ? 0 : 0];
+        (void)j;
+    }
+    if (d.n_phi > 0) {
+        const bool any_blocked = cfg.is_hblocked || cfg.is_pblocked;
+        nsw_phi = any_blocked ? 2 * d.P : 1;
+    }
+    for (uint32_t k = 0; k < n_iter; ++k, ++next_iter) {
+        for (int sw = 0; sw < nsw_phi; ++sw)
+            for (int j = 0; j < d.nchain; ++j)
+                HIPCHECK(ggdmc_internal_launch_plain_phi(&d, next_iter, sw, j, hyper_rows, hyper_off, stream));
+        if (d.S > 0)
+            for (int sw = 0; sw < nsweep_sub; ++sw)
+                for (int j = 0; j < d.nchain; ++j)
+                    HIPCHECK(ggdmc_internal_launch_plain_subject(&d, next_iter, sw, j, stream));
+        HIPCHECK(ggdmc_internal_launch_store(&d, next_iter, stream));
+    }
+}
+
+/* Picks (trials per tile, trials per lane) by timing a few real iterations
+ * of each candidate from a snapshot of the state, which is then restored. */
+void ggdmc_session::autotune() {
+    if (d.S == 0) return;
+    std::vector<int> cands{0};
+    for (int t = 64; t < max_ntr; t *= 2) cands.push_back(t);
+    const size_t nth = (size_t)d.S * d.nchain * d.P, nch = (size_t)d.S * d.nchain;
+    const size_t nph = (size_t)d.n_phi * d.nchain * 2 * d.P, npc = (size_t)d.n_phi * d.nchain;
+    double *snap = pool.alloc<double>(nth + 2 * nch + nph + 2 * npc);
+    unsigned long long ev_snap = 0;
+    auto save = [&]() {
+        HIPCHECK(hipMemcpyAsync(snap, d.theta, nth * 8, hipMemcpyDeviceToDevice, stream));
+        HIPCHECK(hipMemcpyAsync(snap + nth, d.lp, nch * 8, hipMemcpyDeviceToDevice, stream));
+        HIPCHECK(hipMemcpyAsync(snap + nth + nch, d.ll, nch * 8, hipMemcpyDeviceToDevice, stream));
+        if (nph) {
+            HIPCHECK(hipMemcpyAsync(snap + nth + 2 * nch, d.phi, nph * 8, hipMemcpyDeviceToDevice, stream));
+            HIPCHECK(hipMemcpyAsync(snap + nth + 2 * nch + nph, d.phi_lp, npc * 8, hipMemcpyDeviceToDevice, stream));
+            HIPCHECK(hipMemcpyAsync(snap + nth + 2 * nch + nph + npc, d.phi_ll, npc * 8, hipMemcpyDeviceToDevice, stream));
+        }
+        HIPCHECK(hipMemcpyAsync(&ev_snap, d.evals, 8, hipMemcpyDeviceToHost, stream));
+        HIPCHECK(hipStreamSynchronize(stream));
+    };
+    auto restore = [&]() {
+        HIPCHECK(hipMemcpyAsync(d.theta, snap, nth * 8, hipMemcpyDeviceToDevice, stream));
+        HIPCHECK(hipMemcpyAsync(d.lp, snap + nth, nch * 8, hipMemcpyDeviceToDevice, stream));
+        HIPCHECK(hipMemcpyAsync(d.ll, snap + nth + nch, nch * 8, hipMemcpyDeviceToDevice, stream));
+        if (nph) {
+            HIPCHECK(hipMemcpyAsync(d.phi, snap + nth + 2 * nch, nph * 8, hipMemcpyDeviceToDevice, stream));
+            HIPCHECK(hipMemcpyAsync(d.phi_lp, snap + nth + 2 * nch + nph, npc * 8, hipMemcpyDeviceToDevice, stream));
+            HIPCHECK(hipMemcpyAsync(d.phi_ll, snap + nth + 2 * nch + nph + npc, npc * 8, hipMemcpyDeviceToDevice, stream));
+        }
+        HIPCHECK(hipMemcpyAsync(d.evals, &ev_snap, 8, hipMemcpyHostToDevice, stream));
+        HIPCHECK(hipStreamSynchronize(stream));
+    };
+    save();
+    const uint32_t it0 = next_iter;
+    float best = 1e30f;
+    int best_t = 0, best_l = 2;
+    const int reps = 3;
+    for (int t : cands) {
+        set_tiles(t);
+        if (d.n_tiles > 65536) continue;
+        for (int l : {1, 2}) {
+            tpl = l;
+            next_iter = it0;
+            run_batched(1); /* warm-up */
+            HIPCHECK(hipEventRecord(ev0, stream));
+            run_batched(reps);
+            HIPCHECK(hipEventRecord(ev1, stream));
+            HIPCHECK(hipEventSynchronize(ev1));
+            float ms = 0.f;
+            HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
+            restore();
+            if (ms < best) {
+                best = ms;
+                best_t = t;
+                best_l = l;
+            }
+        }
+    }
+    next_iter = it0;
+    tpl = best_l;
+    set_tiles(best_t);
+}
+
+static int fail(const std::exception &e) {
+    if (const Error *er = dynamic_cast<const Error *>(&e)) {
+        g_error = er->what();
+        return er->code;
+    }
+    g_error = e.what();
+    return GGDMC_EINVAL;
+}
+
+/* ================================================================= C ABI */
+extern "C" {
+
+const char *ggdmc_last_error(void) { return g_error.c_str(); }
+
+int ggdmc_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    int ok = 0;
+    for (int i = 0; i < n; ++i) {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, i) == hipSuccess && std::strncmp(p.gcnArchName, "gfx950", 6) == 0) ++ok;
+    }
+    return ok;
+}
+
+int ggdmc_session_create(const ggdmc_config *cfg, const ggdmc_model *model, const ggdmc_job *jobs,
+                         int32_t njobs, ggdmc_session **out) {
+    if (!out) {
+        g_error = "out is null";
+        return GGDMC_EINVAL;
+    }
+    *out = nullptr;
+    if (ggdmc_device_count() < 1) {
+        g_error = "no gfx950 device visible";
+        return GGDMC_ENODEV;
+    }
+    auto s = std::make_unique<ggdmc_session>();
+    try {
+        s->build(cfg, model, jobs, njobs);
+        if (s->mode != GGDMC_FP64_PLAIN) {
+            if (cfg->trials_per_tile == 0) s->autotune();
+            const uint32_t total = s->last_iter;
+            const int K = (int)std::min<uint32_t>(64u, total);
+            s->capture(K);
+        }
+        HIPCHECK(hipStreamSynchronize(s->stream));
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+    *out = s.release();
+    return GGDMC_OK;
+}
+
+int ggdmc_session_advance(ggdmc_session *s, int32_t n_iter) {
+    if (!s || n_iter < 0) {
+        g_error = "bad session / n_iter";
+        return GGDMC_EINVAL;
+    }
+    try {
+        const uint32_t left = s->next_iter <= s->last_iter ? s->last_iter - s->next_iter + 1 : 0;
+        const uint32_t n = std::min<uint32_t>((uint32_t)n_iter, left);
+        HIPCHECK(hipEventRecord(s->ev0, s->stream));
+        if (s->mode == GGDMC_FP64_PLAIN) s->run_plain(n);
+        else s->run_batched(n);
+        HIPCHECK(hipEventRecord(s->ev1, s->stream));
+        HIPCHECK(hipEventSynchronize(s->ev1));
+        HIPCHECK(hipEventElapsedTime(&s->last_ms, s->ev0, s->ev1));
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+    return GGDMC_OK;
+}
+
+int ggdmc_session_elapsed_ms(ggdmc_session *s, float *ms) {
+    if (!s || !ms) return GGDMC_EINVAL;
+    *ms = s->last_ms;
+    return GGDMC_OK;
+}
+
+int ggdmc_session_tile(ggdmc_session *s, int32_t *trials_per_tile, int32_t *n_tiles) {
+    if (!s) return GGDMC_EINVAL;
+    if (trials_per_tile) *trials_per_tile = s->tpt;
+    if (n_tiles) *n_tiles = s->d.n_tiles;
+    return GGDMC_OK;
+}
+
+int ggdmc_session_trial_evals(ggdmc_session *s, double *evals) {
+    if (!s || !evals) return GGDMC_EINVAL;
+    try {
+        unsigned long long v = 0;
+        HIPCHECK(hipMemcpyAsync(&v, s->d.evals, 8, hipMemcpyDeviceToHost, s->stream));
+        HIPCHECK(hipStreamSynchronize(s->stream));
+        *evals = (double)v;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+    return GGDMC_OK;
+}
+
+int ggdmc_session_profile(ggdmc_session *s, int32_t n_iter, double *subject_kernel_ms,
+                          int32_t *n_launches) {
+    if (!s || !subject_kernel_ms || n_iter < 1) return GGDMC_EINVAL;
+    try {
+        if (s->mode == GGDMC_FP64_PLAIN || s->d.S == 0) {
+            g_error = "profile needs a batched session with subjects";
+            return GGDMC_EINVAL;
+        }
+        const uint32_t left = s->next_iter <= s->last_iter ? s->last_iter - s->next_iter + 1 : 0;
+        const uint32_t n = std::min<uint32_t>((uint32_t)n_iter, left);
+        const int per_iter = s->nsweep_sub * (s->fused_groups ? 1 : s->d.G);
+        std::vector<hipEvent_t> ev(2 * (size_t)n * per_iter);
+        for (auto &e : ev) HIPCHECK(hipEventCreate(&e));
+        const int lk = s->mode == GGDMC_FP64_BATCHED ? GGDMC_FP64_BATCHED : GGDMC_MIXED_BATCHED;
+        size_t q = 0;
+        for (uint32_t k = 0; k < n; ++k) {
+            set_base_kernel<<<1, 1, 0, s->stream>>>(s->iter_base, s->next_iter);
+            HIPCHECK(hipGetLastError());
+            if (s->d.n_phi > 0) HIPCHECK(ggdmc_internal_launch_phi(&s->d, 0, s->phi_lds, s->stream));
+            for (int sw = 0; sw < s->nsweep_sub; ++sw) {
+                const int ng = s->fused_groups ? 1 : s->d.G;
+                for (int g = 0; g < ng; ++g) {
+                    SweepArgs a{0, sw, s->fused_groups ? 0 : g, s->fused_groups ? s->d.G : g + 1,
+                                (sw == s->nsweep_sub - 1 && g == ng - 1), s->chain_batch};
+                    HIPCHECK(hipEventRecord(ev[q++], s->stream));
+                    HIPCHECK(ggdmc_internal_launch_sweep(&s->d, &a, lk, s->tpl, s->sweep_lds, s->stream));
+                    HIPCHECK(hipEventRecord(ev[q++], s->stream));
+                }
+            }
+            s->next_iter += 1;
+        }
+        HIPCHECK(hipStreamSynchronize(s->stream));
+        double tot = 0.0;
+        for (size_t i = 0; i < q; i += 2) {
+            float ms = 0.f;
+            HIPCHECK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+            tot += ms;
+        }
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        *subject_kernel_ms = q ? tot / (double)(q / 2) : 0.0;
+        if (n_launches) *n_launches = (int32_t)(q / 2);
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+    return GGDMC_OK;
+}
+
+int ggdmc_session_download(ggdmc_session *s) {
+    if (!s) return GGDMC_EINVAL;
+    try {
+        const size_t nmc = (size_t)s->cfg.nmc, n = (size_t)s->d.nchain, P = (size_t)s->d.P;
+        HIPCHECK(hipStreamSynchronize(s->stream));
+        for (size_t ji = 0; ji < s->jobs.size(); ++ji) {
+            const ggdmc_job &j = s->jobs[ji];
+            const int f = s->phi_of_job[ji];
+            if (f >= 0) {
+                ggdmc_samples *o = j.phi_out;
+                HIPCHECK(hipMemcpy(o->theta, s->d.phi_out_theta + (size_t)f * nmc * n * 2 * P,
+                                   sizeof(double) * nmc * n * 2 * P, hipMemcpyDeviceToHost));
+                HIPCHECK(hipMemcpy(o->lp, s->d.phi_out_lp + (size_t)f * nmc * n, sizeof(double) * nmc * n,
+                                   hipMemcpyDeviceToHost));
+                HIPCHECK(hipMemcpy(o->ll, s->d.phi_out_ll + (size_t)f * nmc * n, sizeof(double) * nmc * n,
+                                   hipMemcpyDeviceToHost));
+            }
+            if (j.kind == GGDMC_KIND_HYPER) continue;
+            const int ns = j.kind == GGDMC_KIND_SUBJECT ? 1 : j.nsubject;
+            for (int k = 0; k < ns; ++k) {
+                const size_t sg = (size_t)s->subj_of_job[ji] + k;
+                ggdmc_samples *o = &j.subject_outs[k];
+                HIPCHECK(hipMemcpy(o->theta, s->d.out_theta + sg * nmc * n * P, sizeof(double) * nmc * n * P,
+                                   hipMemcpyDeviceToHost));
+                HIPCHECK(hipMemcpy(o->lp, s->d.out_lp + sg * nmc * n, sizeof(double) * nmc * n,
+                                   hipMemcpyDeviceToHost));
+                HIPCHECK(hipMemcpy(o->ll, s->d.out_ll + sg * nmc * n, sizeof(double) * nmc * n,
+                                   hipMemcpyDeviceToHost));
+            }
+        }
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+    return GGDMC_OK;
+}
+
+void ggdmc_session_destroy(ggdmc_session *s) { delete s; }
+
+int ggdmc_run_batch(const ggdmc_config *cfg, const ggdmc_model *model, const ggdmc_job *jobs,
+                    int32_t njobs) {
+    ggdmc_session *s = nullptr;
+    int rc = ggdmc_session_create(cfg, model, jobs, njobs, &s);
+    if (rc != GGDMC_OK) return rc;
+    rc = ggdmc_session_advance(s, (int32_t)s->last_iter);
+    if (rc == GGDMC_OK) rc = ggdmc_session_download(s);
+    ggdmc_session_destroy(s);
+    return rc;
+}
+
+int ggdmc_run_subject(const ggdmc_config *cfg, const ggdmc_model *model, const ggdmc_prior *p_prior,
+                      const ggdmc_subject *data, const ggdmc_state *start, uint64_t seed,
+                      ggdmc_samples *out) {
+    ggdmc_job j{};
+    j.kind = GGDMC_KIND_SUBJECT;
+    j.seed = seed;
+    j.p_prior = p_prior;
+    j.nsubject = 1;
+    j.subjects = data;
+    j.subject_starts = start;
+    j.subject_outs = out;
+    return ggdmc_run_batch(cfg, model, &j, 1);
+}
+
+int ggdmc_run_hyper(const ggdmc_config *cfg, const ggdmc_prior *p_prior, const ggdmc_prior *h_prior,
+                    int32_t nsubject, const double *hyper_data, const ggdmc_state *start, uint64_t seed,
+                    ggdmc_samples *out) {
+    ggdmc_job j{};
+    j.kind = GGDMC_KIND_HYPER;
+    j.seed = seed;
+    j.p_prior = p_prior;
+    j.h_prior = h_prior;
+    j.nsubject = nsubject;
+    j.hyper_data = hyper_data;
+    j.phi_start = start;
+    j.phi_out = out;
+    return ggdmc_run_batch(cfg, nullptr, &j, 1);
+}
+
+int ggdmc_run(const ggdmc_config *cfg, const ggdmc_model *model, const ggdmc_prior *p_prior,
+              const ggdmc_prior *h_prior, int32_t nsubject, const ggdmc_subject *subjects,
+              const ggdmc_state *phi_start, const ggdmc_state *subject_starts, uint64_t seed,
+              ggdmc_samples *phi_out, ggdmc_samples *subject_outs) {
+    ggdmc_job j{};
+    j.kind = GGDMC_KIND_HIER;
+    j.seed = seed;
+    j.p_prior = p_prior;
+    j.h_prior = h_prior;
+    j.nsubject = nsubject;
+    j.subjects = subjects;
+    j.phi_start = phi_start;
+    j.subject_starts = subject_starts;
+    j.phi_out = phi_out;
+    j.subject_outs = subject_outs;
+    return ggdmc_run_batch(cfg, model, &j, 1);
+}
+
+int ggdmc_loglike(const ggdmc_model *model, const ggdmc_subject *data, const double *theta, int32_t n,
+                  int32_t precision, double *ll_out) {
+    if (ggdmc_device_count() < 1) {
+        g_error = "no gfx950 device visible";
+        return GGDMC_ENODEV;
+    }
+    if (n < 1 || !theta || !ll_out) {
+        g_error = "theta / ll_out missing";
+        return GGDMC_EINVAL;
+    }
+    try {
+        require(precision == GGDMC_FP64_PLAIN || precision == GGDMC_MIXED_BATCHED,
+                "loglike precision must be FP64_PLAIN or MIXED_BATCHED");
+        /* one-subject session without sampling: reuse the flattening */
+        std::vector<double> zeros((size_t)model->npar * 3, 0.0), zl(3, 0.0);
+        std::vector<int32_t> dist(model->npar, GGDMC_DIST_UNIF);
+        std::vector<double> p0(model->npar, 0.0), p1(model->npar, 1.0);
+        ggdmc_prior pr{model->npar, dist.data(), p0.data(), p1.data(), p0.data(), p1.data()};
+        ggdmc_state st{zeros.data(), zl.data(), zl.data()};
+        std::vector<double> ot((size_t)model->npar * 3), ol(3), oll(3);
+        ggdmc_samples so{ot.data(), ol.data(), oll.data()};
+        ggdmc_job j{};
+        j.kind = GGDMC_KIND_SUBJECT;
+        j.p_prior = &pr;
+        j.nsubject = 1;
+        j.subjects = data;
+        j.subject_starts = &st;
+        j.subject_outs = &so;
+        ggdmc_config c{};
+        c.nmc = 1;
+        c.thin = 1;
+        c.nchain = 3;
+        c.sweep_groups = 3;
+        c.gamma_precursor = 2.38;
+        c.precision = GGDMC_MIXED_BATCHED;
+        auto s = std::make_unique<ggdmc_session>();
+        s->build(&c, model, &j, 1);
+        double *dth = s->pool.alloc<double>((size_t)n * model->npar);
+        double *dout = s->pool.alloc<double>((size_t)n);
+        HIPCHECK(hipMemcpyAsync(dth, theta, sizeof(double) * n * model->npar, hipMemcpyHostToDevice, s->stream));
+        const size_t lds = ggdmc_internal_sweep_lds(1, model->npar, 1, model->n_cell, model->n_acc, 1);
+        HIPCHECK(ggdmc_internal_launch_loglike(&s->d, dth, n, 0, precision, lds, s->stream, dout));
+        HIPCHECK(hipMemcpyAsync(ll_out, dout, sizeof(double) * n, hipMemcpyDeviceToHost, s->stream));
+        HIPCHECK(hipStreamSynchronize(s->stream));
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+    return GGDMC_OK;
+}
+
+} // extern "C"

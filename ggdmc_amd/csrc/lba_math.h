@@ -1,0 +1,223 @@
+/*
+ * LBA trial log-likelihood arithmetic for the HIP sampler.
+ *
+ * Reference semantics (external ggdmcHeaders/likelihood.h `sumloglike`, called
+ * at src/de.cpp:145,187,604,653; per-trial floor R/phi.R:3-13): for a trial of
+ * cell c with responding accumulator w and each accumulator j with
+ * t = rt - t0_j > 0,
+ *     zs = t*s, chi = b - t*v, z1 = chi/zs, z2 = (chi - A)/zs, den = max(Phi(v/s), 1e-10)
+ *     pdf_j = (v*(Phi(z1)-Phi(z2)) + s*(phi(z2)-phi(z1))) / (A*den)
+ *     F_j   = (1 + (zs*(phi(z2)-phi(z1)) + (chi-A)*Phi(z2) - chi*Phi(z1)) / A) / den
+ *     density = max(pdf_w, 1e-10) * prod_{j != w} (1 - max(F_j, 1e-10))
+ *     term = log(density > 0 ? density : DBL_EPSILON)
+ * (t <= 0 gives pdf = F = 0.)
+ *
+ * Two evaluations are provided:
+ *
+ *  - `naive_*`: the formula above, term by term, in FP64 (the plain FP64 path).
+ *
+ *  - `mixed_*`: the batched path. CDNA4 runs FP64 FMA at the rate of
+ *    non-packed FP32, so all polynomial / rational arithmetic stays FP64 and
+ *    only the transcendental instructions run in FP32 hardware (v_exp_f32,
+ *    v_log_f32, v_rcp_f32) with split arguments:
+ *      * exp(-z^2/2) = 2^(y_hi) * (1 + y_lo*ln2), y = -z^2/2*log2(e) in FP64;
+ *      * 1/x from v_rcp_f32 plus one FP64 Newton step;
+ *      * Q(|z|) = exp(-z^2/2) * R(|z|), R the Mills-ratio factor
+ *        Q(z)*exp(z^2/2), approximated as t*sum_k c_k t^k with t = 1/(1+0.275|z|)
+ *        (coefficients fitted for this library by weighted least squares to
+ *        0.5*erfcx(z/sqrt(2)) on [0, 14]; max relative error 8.2e-10);
+ *      * the small tail of Phi is always the one computed, so
+ *        Phi(z1) - Phi(z2) is taken as Q(z2) - Q(z1) when z2 >= 0;
+ *      * the survivor of a loser is formed from whichever of the untruncated
+ *        CDF G or its complement den - G is small (both written without
+ *        cancellation, see below) and then passed through the reference's own
+ *        FP64 operations F = G/den, 1 - max(F, 1e-10). Where the reference's
+ *        1 - F is quantised by FP64 rounding (survivors of 1e-16 .. 1e-12 of
+ *        sharp accumulators), this reproduces the quantisation; the DBL_EPSILON
+ *        replacement and its sign rule follow from the same FP64 values.
+ *    Untruncated CDF and complement, with Q = 1 - Phi, Pc = Phi(-v/s):
+ *        G       = (zs*(phi2 - phi1) + chi*Q1 - (chi - A)*Q2) / A
+ *        den - G = (Phi2 - Pc) + (chi*(Phi1 - Phi2) + zs*(phi1 - phi2)) / A
+ *    (the second from integrating the drift density over the start point:
+ *    P(0 < drift < (b - a)/t) averaged over a ~ U(0, A)).
+ *    log(x) = e*ln2 + log2(m)*ln2 with the FP32 log2 applied to the mantissa
+ *    only, so its error is absolute ~1e-8 whatever the magnitude of log(x).
+ */
+#ifndef GGDMC_LBA_MATH_H
+#define GGDMC_LBA_MATH_H
+
+#include <cmath>
+#include <cstdint>
+
+#ifdef __HIPCC__
+#define GGDMC_FN __host__ __device__ __forceinline__
+#else
+#define GGDMC_FN inline
+#endif
+
+namespace ggdmc {
+
+constexpr double kFloor = 1e-10;
+constexpr double kLogFloor = -23.025850929940457;          /* log(1e-10)           */
+constexpr double kLogDblEps = -36.04365338911715;          /* log(2.220446e-16)   */
+constexpr double kInvSqrt2Pi = 0.3989422804014327;
+constexpr double kLog2e = 1.4426950408889634;
+constexpr double kLn2 = 0.6931471805599453;
+constexpr double kSqrt1_2 = 0.7071067811865476;
+constexpr double kMillsK = 0.275;
+constexpr int kMillsDeg = 11;
+/* R(z) ~= t * sum_{k=0}^{11} c_k t^k, t = 1/(1 + 0.275 z), z >= 0 */
+constexpr double kMills[kMillsDeg + 1] = {
+    0.10970522021435668,  0.10981321934884514, 0.10017863047558236, 0.09341028453443045,
+    0.022740656309337238, 0.1579382946903426,  -0.252144678841467,  0.39597478558978705,
+    -0.41305660985486853, 0.23823343654297754, -0.0717659160056362, 0.008972676989147887};
+
+/* Per (chain, cell, accumulator) constants, computed once per proposal. */
+struct AccK {
+    double A, b, v, s, t0;
+    double invA, den, inv_den, Pc;
+};
+
+/* ---------------------------------------------------------------- host/device
+ * Policy interface (Ops): float exp2(float), float log2(float), float rcp(float).
+ */
+
+template <class Ops>
+GGDMC_FN double rcp64(double x) {
+    float xf = (float)x;
+    xf = xf < 1e-30f ? 1e-30f : xf;
+    double r = (double)Ops::rcp(xf);
+    return r * (2.0 - x * r);
+}
+
+/* e = exp(-z^2/2), qa = Q(|z|) = upper normal tail beyond |z|. */
+template <class Ops>
+GGDMC_FN void mixed_tail(double z, double &e, double &qa) {
+    const double y = -0.5 * z * z * kLog2e;
+    const float yh = (float)y;
+    float yl = (float)(y - (double)yh);
+    if (!(yh > -1e30f)) yl = 0.0f; /* y = -inf or NaN: keep e = 0 / NaN */
+    e = (double)Ops::exp2(yh) * (1.0 + (double)yl * kLn2);
+    const double a = std::fabs(z);
+    const double t = rcp64<Ops>(1.0 + kMillsK * a);
+    double h = kMills[kMillsDeg];
+#pragma unroll
+    for (int k = kMillsDeg - 1; k >= 0; --k) h = h * t + kMills[k];
+    qa = e * t * h;
+}
+
+/* log(x) for x > 0, FP32 log2 on the mantissa in [sqrt(1/2), sqrt(2)). */
+template <class Ops>
+GGDMC_FN double mixed_log(double x) {
+    int ex;
+    double m = std::frexp(x, &ex); /* x = m * 2^ex, m in [0.5, 1) */
+    if (m < kSqrt1_2) {
+        m *= 2.0;
+        ex -= 1;
+    }
+    const float mf = (float)m;
+    const double r = (m - (double)mf) / (double)mf; /* log(m) = log(mf) + r */
+    return ((double)Ops::log2(mf) + (double)ex) * kLn2 + r;
+}
+
+/* Shared core: Phi differences and the z's of one accumulator at time t > 0. */
+struct Core {
+    double zs, chi, dP, dphi, P2, Q1, Q2;
+};
+
+template <class Ops>
+GGDMC_FN Core mixed_core(const AccK &k, double t) {
+    Core c;
+    c.zs = t * k.s;
+    const double r = rcp64<Ops>(c.zs);
+    c.chi = k.b - t * k.v;
+    const double z1 = c.chi * r;
+    const double z2 = (c.chi - k.A) * r;
+    double e1, q1, e2, q2;
+    mixed_tail<Ops>(z1, e1, q1);
+    mixed_tail<Ops>(z2, e2, q2);
+    c.Q1 = z1 >= 0.0 ? q1 : 1.0 - q1;
+    const double P1 = z1 >= 0.0 ? 1.0 - q1 : q1;
+    c.Q2 = z2 >= 0.0 ? q2 : 1.0 - q2;
+    c.P2 = z2 >= 0.0 ? 1.0 - q2 : q2;
+    c.dP = z2 >= 0.0 ? c.Q2 - c.Q1 : P1 - c.P2; /* Phi(z1) - Phi(z2) */
+    c.dphi = (e2 - e1) * kInvSqrt2Pi;            /* phi(z2) - phi(z1) */
+    return c;
+}
+
+/* Winner: log(max(pdf, 1e-10)); NaN propagates. */
+template <class Ops>
+GGDMC_FN double mixed_winner_logpdf(const AccK &k, double rt) {
+    const double t = rt - k.t0;
+    if (!(t > 0.0)) return t == t ? kLogFloor : t;
+    const Core c = mixed_core<Ops>(k, t);
+    const double pdf = (k.v * c.dP + k.s * c.dphi) * k.invA * k.inv_den;
+    return pdf < kFloor ? kLogFloor : mixed_log<Ops>(pdf);
+}
+
+/* Loser: the reference's FP64 survivor 1 - max(F, 1e-10) (may be <= 0). */
+template <class Ops>
+GGDMC_FN double mixed_loser_survivor(const AccK &k, double rt) {
+    const double t = rt - k.t0;
+    if (!(t > 0.0)) return t == t ? 1.0 - kFloor : t;
+    const Core c = mixed_core<Ops>(k, t);
+    const double G = (c.zs * c.dphi + c.chi * c.Q1 - (c.chi - k.A) * c.Q2) * k.invA;
+    const double Su = (c.P2 - k.Pc) + (c.chi * c.dP - c.zs * c.dphi) * k.invA;
+    const double Gd = (G * k.inv_den < 0.5) ? G : (k.den - Su);
+    const double F = Gd * k.inv_den;
+    return 1.0 - (F > kFloor ? F : (F == F ? kFloor : F));
+}
+
+/* Accumulates one survivor factor into (log |prod|, sign, zero) state. */
+GGDMC_FN void fold_survivor(double S, double logS, double &acc, int &neg, bool &zero) {
+    if (S == 0.0) zero = true;
+    if (S < 0.0) neg ^= 1;
+    acc += logS;
+}
+
+/* ------------------------------------------------------------- naive FP64 */
+template <class Ops>
+GGDMC_FN void naive_pdf_cdf(const AccK &k, double t, double &pdf, double &cdf) {
+    if (!(t > 0.0)) {
+        pdf = t == t ? 0.0 : t;
+        cdf = pdf;
+        return;
+    }
+    const double zs = t * k.s;
+    const double chi = k.b - t * k.v;
+    const double chimA = chi - k.A;
+    const double z1 = chi / zs, z2 = chimA / zs;
+    const double P1 = Ops::pnorm(z1), P2 = Ops::pnorm(z2);
+    const double d1 = Ops::dnorm(z1), d2 = Ops::dnorm(z2);
+    pdf = (k.v * (P1 - P2) + k.s * (d2 - d1)) / (k.A * k.den);
+    cdf = (1.0 + (zs * (d2 - d1) + chimA * P2 - chi * P1) / k.A) / k.den;
+}
+
+/* ----------------------------------------------------------- per (cell, acc)
+ * Resolves the six families from theta / constants and fills AccK.
+ * par_src / par_const are [n_acc * 6] for the cell. Returns false if st0 != 0
+ * (the path implements st0 = 0 only). Phi(x) = 0.5*erfc(-x/sqrt(2)).
+ */
+template <class Ops>
+GGDMC_FN bool make_acck(const double *theta, const int32_t *src6, const double *const6,
+                        AccK &k) {
+    double f[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) f[i] = src6[i] >= 0 ? theta[src6[i]] : const6[i];
+    k.A = f[0];
+    k.b = f[0] + f[1];
+    k.v = f[2];
+    k.s = f[3];
+    k.t0 = f[5];
+    k.invA = 1.0 / k.A;
+    const double x = k.v / k.s;
+    const double Pv = 0.5 * Ops::erfc(-x * kSqrt1_2);
+    k.den = Pv > kFloor ? Pv : (Pv == Pv ? kFloor : Pv);
+    k.inv_den = 1.0 / k.den;
+    k.Pc = 0.5 * Ops::erfc(x * kSqrt1_2);
+    return f[4] == 0.0;
+}
+
+} // namespace ggdmc
+
+#endif

@@ -1,0 +1,174 @@
+/*
+ * libggdmc_hip: MI355X (gfx950) DE-MCMC engine for LBA models.
+ *
+ * C ABI of the drop-in boundary that replaces the Rcpp exports of ggdmc's
+ * src/de2R.cpp (run_subject :8-23, run_hyper :30-47, run :123-171), which
+ * R reaches through .Call stubs in R/RcppExports.R:6-7,12-13,81-82 under
+ * StartSampling* / RestartSampling* (R/sampling.R:247,308,446,507,615,681).
+ *
+ * Conventions
+ *  - Plain pointers and sizes; every array is caller-owned host memory in R's
+ *    column-major layout (theta [P x nchain x nmc] is theta[p + P*(c + nchain*k)]).
+ *  - Start states are one column: theta [P x nchain], lp [nchain], ll [nchain]
+ *    (the Rcpp shim passes column 1 of initialise_theta/initialise_phi output,
+ *    or the last stored column of a previous run to restart).
+ *  - Outputs: column 0 is the start state; column k holds the state after
+ *    iteration k*thin (nmc*thin - 1 iterations are run).
+ *  - Return value 0 on success, a negative GGDMC_E* code otherwise; the
+ *    message of the last failure on the calling thread is ggdmc_last_error().
+ *  - The library initialises HIP on first use and is not fork-safe afterwards:
+ *    run independent runs through ggdmc_run_batch instead of forked processes.
+ */
+#ifndef GGDMC_HIP_H
+#define GGDMC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GGDMC_OK 0
+#define GGDMC_EINVAL -1   /* invalid argument / inconsistent sizes */
+#define GGDMC_EHIP -2     /* HIP runtime failure */
+#define GGDMC_ENOMEM -3   /* device allocation failed */
+#define GGDMC_ENODEV -4   /* no usable gfx950 device */
+
+/* Arithmetic of the likelihood. */
+#define GGDMC_FP64_PLAIN 0   /* reference arithmetic and sweep order; one launch per chain step */
+#define GGDMC_MIXED_BATCHED 1 /* FP32 transcendentals + FP64 sums; batched, graph-captured */
+#define GGDMC_FP64_BATCHED 2 /* reference arithmetic inside the batched kernels */
+
+/* Run kinds (which reference export a job mirrors). */
+#define GGDMC_KIND_SUBJECT 0 /* run_subject: one subject, fixed prior */
+#define GGDMC_KIND_HYPER 1   /* run_hyper: population parameters of fixed subject vectors */
+#define GGDMC_KIND_HIER 2    /* run: population level + subjects */
+
+/* Prior distributions (ggdmcPrior dist_id). */
+#define GGDMC_DIST_TNORM 1
+#define GGDMC_DIST_UNIF 6
+
+/* S4 de_input + theta_input fields (R/model-class.R:36-57, :1288-1313)
+ * plus the engine's own knobs. */
+typedef struct {
+    int32_t nmc, thin, nchain;
+    double pop_migration_prob, sub_migration_prob;
+    double gamma_precursor; /* 2.38 */
+    double rp;              /* 0.001 */
+    int32_t is_hblocked, is_pblocked;
+    int32_t precision;      /* GGDMC_FP64_PLAIN / GGDMC_MIXED_BATCHED / GGDMC_FP64_BATCHED */
+    int32_t sweep_groups;   /* chain groups per sweep: 0 = default (nchain for
+                               FP64_PLAIN, 2 otherwise); nchain = the reference's
+                               sequential in-place order */
+    int32_t trials_per_tile; /* 0 = autotuned */
+    uint64_t iter_offset;   /* Philox iteration offset (restart continues a stream) */
+} ggdmc_config;
+
+/* Flattened LBA design: family f (0 A, 1 B, 2 mean_v, 3 sd_v, 4 st0, 5 t0) of
+ * accumulator j in cell c is theta[par_src[(c*n_acc+j)*6+f]], or
+ * par_const[...] when par_src is -1 (derived from model@model_boolean,
+ * model@constants; winner[c] = dmi@node_1_index[c, 1] - 0-based). */
+typedef struct {
+    int32_t n_acc, n_cell, npar;
+    const int32_t *winner;
+    const int32_t *par_src;
+    const double *par_const;
+} ggdmc_model;
+
+typedef struct { /* one entry per parameter, p_prior / h_prior of an S4 prior */
+    int32_t npar;
+    const int32_t *dist;
+    const double *p0, *p1, *lower, *upper;
+} ggdmc_prior;
+
+typedef struct { /* dmi@data: cell-sorted RTs, cell_off[n_cell + 1] */
+    int64_t ntrial;
+    const double *rt;
+    const int64_t *cell_off;
+} ggdmc_subject;
+
+typedef struct {
+    const double *theta, *lp, *ll;
+} ggdmc_state;
+
+typedef struct {
+    double *theta, *lp, *ll;
+} ggdmc_samples;
+
+typedef struct {
+    int32_t kind;            /* GGDMC_KIND_* */
+    uint64_t seed;           /* Philox key of this run */
+    int32_t nparameter_de;   /* de_input@nparameter (gamma = precursor/sqrt(2*this)); 0 = P (subject) / 2P */
+    const ggdmc_prior *p_prior; /* SUBJECT: the prior; HIER/HYPER: tnorm bounds of the population */
+    const ggdmc_prior *h_prior; /* HIER/HYPER: prior of the 2P population parameters */
+    int32_t nsubject;
+    const ggdmc_subject *subjects;   /* SUBJECT (1) / HIER (nsubject) */
+    const double *hyper_data;        /* HYPER: [nsubject x P] row-major subject vectors */
+    const ggdmc_state *phi_start;    /* HYPER / HIER */
+    const ggdmc_state *subject_starts; /* SUBJECT / HIER: [nsubject] */
+    ggdmc_samples *phi_out;          /* HYPER / HIER */
+    ggdmc_samples *subject_outs;     /* SUBJECT / HIER: [nsubject] */
+} ggdmc_job;
+
+/* Replaces run_subject (src/de2R.cpp:8-23 -> de_class::run_chains, src/de.cpp:201-242). */
+int ggdmc_run_subject(const ggdmc_config *cfg, const ggdmc_model *model,
+                      const ggdmc_prior *p_prior, const ggdmc_subject *data,
+                      const ggdmc_state *start, uint64_t seed, ggdmc_samples *out);
+
+/* Replaces run_hyper (src/de2R.cpp:30-47): the data are subject parameter
+ * vectors hyper_data [nsubject x P] (row-major), the likelihood of the 2P
+ * population parameters is de_class::sumloghlike (src/de.cpp:245-270). */
+int ggdmc_run_hyper(const ggdmc_config *cfg, const ggdmc_prior *p_prior,
+                    const ggdmc_prior *h_prior, int32_t nsubject, const double *hyper_data,
+                    const ggdmc_state *start, uint64_t seed, ggdmc_samples *out);
+
+/* Replaces run (src/de2R.cpp:123-171 -> de_class::run_hchains, src/de.cpp:272-383). */
+int ggdmc_run(const ggdmc_config *cfg, const ggdmc_model *model, const ggdmc_prior *p_prior,
+              const ggdmc_prior *h_prior, int32_t nsubject, const ggdmc_subject *subjects,
+              const ggdmc_state *phi_start, const ggdmc_state *subject_starts, uint64_t seed,
+              ggdmc_samples *phi_out, ggdmc_samples *subject_outs);
+
+/* Many independent runs (parallel_lapply's ncore runs, R/sampling.R:13-121;
+ * recovery fits) in one launch sequence. All jobs share cfg and model. */
+int ggdmc_run_batch(const ggdmc_config *cfg, const ggdmc_model *model, const ggdmc_job *jobs,
+                    int32_t njobs);
+
+/* Likelihood::sumloglike (external ggdmcHeaders, called at src/de.cpp:145) /
+ * ggdmcLikelihood::compute_subject_likelihood + .sumlog (R/phi.R:3-13,176) for
+ * n parameter vectors theta [P x n] of one subject. precision: GGDMC_FP64_PLAIN
+ * (reference arithmetic) or GGDMC_MIXED_BATCHED. */
+int ggdmc_loglike(const ggdmc_model *model, const ggdmc_subject *data, const double *theta,
+                  int32_t n, int32_t precision, double *ll_out);
+
+/* Sessions: a batch whose inputs and state stay resident in HBM, advanced in
+ * steps (restart without re-upload; timing of the device-resident loop). */
+typedef struct ggdmc_session ggdmc_session;
+int ggdmc_session_create(const ggdmc_config *cfg, const ggdmc_model *model,
+                         const ggdmc_job *jobs, int32_t njobs, ggdmc_session **out);
+/* Runs the next n_iter iterations; stored columns follow the thinning rule. */
+int ggdmc_session_advance(ggdmc_session *s, int32_t n_iter);
+/* Device time in ms of the last advance (HIP events on the session stream). */
+int ggdmc_session_elapsed_ms(ggdmc_session *s, float *ms);
+/* Copies every job's stored samples into its *_out buffers. */
+int ggdmc_session_download(ggdmc_session *s);
+/* Average device time (ms) of the subject-level kernel over n_iter
+ * iterations run launch by launch with events around each launch, and the
+ * number of those launches. */
+int ggdmc_session_profile(ggdmc_session *s, int32_t n_iter, double *subject_kernel_ms,
+                          int32_t *n_launches);
+/* Trial-likelihood evaluations (trial x proposal) done by the subject-level
+ * kernels since creation, counting every proposal whose prior is finite. */
+int ggdmc_session_trial_evals(ggdmc_session *s, double *evals);
+void ggdmc_session_destroy(ggdmc_session *s);
+
+/* Autotuned trials per tile for a session (after create). */
+int ggdmc_session_tile(ggdmc_session *s, int32_t *trials_per_tile, int32_t *n_tiles);
+
+const char *ggdmc_last_error(void);
+/* Number of visible gfx950 devices (0 without a GPU; never fails). */
+int ggdmc_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,165 @@
+"""Sampler parity: with FP64 arithmetic the HIP sampler follows the oracle's
+trajectory draw for draw (same Philox streams, same chain-group order), for
+the plain path (the reference's sequential order) and the batched kernels;
+the mixed-precision batched path keeps stored log-likelihoods within the
+per-proposal tolerance of an FP64 recomputation."""
+import numpy as np
+import pytest
+
+import fixtures as fx
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+RT = 1e-9   # trajectory tolerance (FP64 vs FP64: summation order only)
+
+
+def _cfg(g, **kw):
+    base = dict(nmc=6, nchain=12, thin=1, sub_migration_prob=0.25)
+    base.update(kw)
+    return g.make_config(**base)
+
+
+def _ocfg(fix, cfg, nparameter_de=None, seed=9032):
+    return O.config(nmc=cfg.nmc, nchain=cfg.nchain, thin=cfg.thin,
+                    nparameter_de=nparameter_de or fix.P,
+                    pop_migration_prob=cfg.pop_migration_prob,
+                    sub_migration_prob=cfg.sub_migration_prob,
+                    gamma_precursor=cfg.gamma_precursor, rp=cfg.rp,
+                    is_hblocked=bool(cfg.is_hblocked), is_pblocked=bool(cfg.is_pblocked),
+                    sweep_groups=cfg.sweep_groups or cfg.nchain, seed=seed,
+                    iter_offset=cfg.iter_offset)
+
+
+def _subject_case(g, precision, groups, tpt=0, blocked=False, mig=0.25, thin=1, nmc=6):
+    fix = fx.load("lba_data6")
+    n = 12
+    start = fx.near_start(fix, n, seed=7)
+    cfg = _cfg(g, nchain=n, precision=precision, sweep_groups=groups, trials_per_tile=tpt,
+               is_pblocked=blocked, sub_migration_prob=mig, thin=thin, nmc=nmc)
+    post = g.run_subject(cfg, fix.model, fix.sub_prior, fix.sub_data, start, seed=9032)
+    ocfg = _ocfg(fix, cfg)
+    ocfg.sweep_groups = groups or n
+    oth, olp, oll = O.run_subject(ocfg, fix.omodel, fix.oprior, fix.sub_data.rt,
+                                  fix.sub_data.cell_off, *start)
+    return post, oth, olp, oll, start
+
+
+def _same(post, oth, olp, oll):
+    assert np.allclose(post.theta, oth, rtol=RT, atol=1e-12)
+    assert np.allclose(post.summed_log_prior, olp, rtol=RT, atol=1e-12)
+    assert np.allclose(post.log_likelihoods, oll, rtol=RT, atol=1e-12)
+
+
+def test_plain_fp64_follows_oracle(gpu):
+    post, oth, olp, oll, start = _subject_case(gpu, gpu.FP64_PLAIN, 12)
+    _same(post, oth, olp, oll)
+    assert not np.allclose(oth[:, :, -1], start[0])     # the chains moved
+
+
+def test_batched_fp64_two_groups_follows_oracle(gpu):
+    _same(*_subject_case(gpu, gpu.FP64_BATCHED, 2)[:4])
+
+
+def test_batched_fp64_sequential_groups_follows_oracle(gpu):
+    _same(*_subject_case(gpu, gpu.FP64_BATCHED, 12)[:4])
+
+
+def test_batched_multitile_follows_oracle(gpu):
+    # 768 trials in tiles of 128: one launch per group, last-arriving tile accepts
+    _same(*_subject_case(gpu, gpu.FP64_BATCHED, 2, tpt=128)[:4])
+
+
+def test_blocked_follows_oracle(gpu):
+    _same(*_subject_case(gpu, gpu.FP64_PLAIN, 12, blocked=True, nmc=3)[:4])
+    _same(*_subject_case(gpu, gpu.FP64_BATCHED, 3, blocked=True, nmc=3)[:4])
+
+
+def test_thinning_follows_oracle(gpu):
+    _same(*_subject_case(gpu, gpu.FP64_BATCHED, 2, thin=3, nmc=4)[:4])
+
+
+def _hier_case(g, precision, groups, S=6, n=10, nmc=5, tpt=0, blocked=False):
+    fix = fx.load("lba_data6")
+    (phi, plp, pll), subs = fx.hier_start(fix, S, n, seed=11)
+    cfg = g.make_config(nmc=nmc, nchain=n, pop_migration_prob=0.3, sub_migration_prob=0.2,
+                        precision=precision, sweep_groups=groups, trials_per_tile=tpt,
+                        is_hblocked=blocked, is_pblocked=blocked)
+    datas = [fix.pop_data(s) for s in range(S)]
+    r = g.run(cfg, fix.model, fix.pop_prior, fix.hyper_prior, datas, (phi, plp, pll), subs,
+              seed=4242)
+    ocfg = _ocfg(fix, cfg, nparameter_de=2 * fix.P, seed=4242)
+    ocfg.sweep_groups = groups or n
+    rt = np.concatenate([d.rt for d in datas])
+    so = np.concatenate([[0], np.cumsum([len(d.rt) for d in datas])])
+    co = np.stack([d.cell_off for d in datas])
+    o = O.run_hier(ocfg, fix.omodel, fx.oprior(fix.pop_prior), fx.oprior(fix.hyper_prior), rt, so,
+                   co, phi, plp, pll, np.stack([s[0] for s in subs]),
+                   np.stack([s[1] for s in subs]), np.stack([s[2] for s in subs]))
+    return r, o
+
+
+def _same_hier(r, o):
+    pth, plp, pll, sth, slp, sll = o
+    assert np.allclose(r["phi"].theta, pth, rtol=RT, atol=1e-12)
+    assert np.allclose(r["phi"].summed_log_prior, plp, rtol=RT, atol=1e-12)
+    assert np.allclose(r["phi"].log_likelihoods, pll, rtol=1e-8, atol=1e-9)
+    for s, post in enumerate(r["subject_theta"]):
+        assert np.allclose(post.theta, sth[s], rtol=RT, atol=1e-12)
+        assert np.allclose(post.summed_log_prior, slp[s], rtol=RT, atol=1e-12)
+        assert np.allclose(post.log_likelihoods, sll[s], rtol=RT, atol=1e-12)
+
+
+def test_hier_plain_follows_oracle(gpu):
+    _same_hier(*_hier_case(gpu, gpu.FP64_PLAIN, 10))
+
+
+def test_hier_batched_follows_oracle(gpu):
+    _same_hier(*_hier_case(gpu, gpu.FP64_BATCHED, 2))
+    _same_hier(*_hier_case(gpu, gpu.FP64_BATCHED, 2, tpt=256))
+
+
+def test_hier_blocked_follows_oracle(gpu):
+    _same_hier(*_hier_case(gpu, gpu.FP64_BATCHED, 2, nmc=2, blocked=True))
+
+
+def test_hyper_follows_oracle(gpu):
+    fix = fx.load("lba_data6")
+    S, n, P = 32, 10, fix.P
+    rng = np.random.default_rng(2)
+    data = fx.truth_vector(fix)[None, :] * np.exp(rng.normal(0, 0.1, (S, P)))
+    (phi, _, _), _ = fx.hier_start(fix, 0, n, seed=3)
+    hp = fx.oprior(fix.hyper_prior)
+    pp = fx.oprior(fix.pop_prior)
+    plp = np.array([O.sumlogprior(hp, phi[:, c]) for c in range(n)])
+    pll = np.array([O.sumloghlike(pp, phi[:, c], data) for c in range(n)])
+    for prec, G in ((gpu.FP64_PLAIN, n), (gpu.FP64_BATCHED, 2)):
+        cfg = gpu.make_config(nmc=8, nchain=n, sub_migration_prob=0.2, precision=prec,
+                              sweep_groups=G)
+        post = gpu.run_hyper(cfg, fix.pop_prior, fix.hyper_prior, data, (phi, plp, pll), seed=77)
+        ocfg = _ocfg(fix, cfg, nparameter_de=2 * P, seed=77)
+        ocfg.sweep_groups = G
+        oth, olp, oll = O.run_hyper(ocfg, pp, hp, data, phi, plp, pll)
+        assert np.allclose(post.theta, oth, rtol=RT, atol=1e-12)
+        assert np.allclose(post.log_likelihoods, oll, rtol=1e-8, atol=1e-9)
+
+
+def test_mixed_hier_stored_ll_within_tolerance(gpu):
+    fix = fx.load("lba_data6")
+    S, n = 8, 12
+    (phi, plp, pll), subs = fx.hier_start(fix, S, n, seed=5)
+    cfg = gpu.make_config(nmc=20, nchain=n, precision=gpu.MIXED_BATCHED, sweep_groups=2)
+    r = gpu.run(cfg, fix.model, fix.pop_prior, fix.hyper_prior,
+                [fix.pop_data(s) for s in range(S)], (phi, plp, pll), subs, seed=99)
+    worst = 0.0
+    moved = 0
+    for s, post in enumerate(r["subject_theta"]):
+        d = fix.pop_data(s)
+        for k in (5, 19):
+            for c in range(n):
+                ref = O.lba_loglike(fix.omodel, d.rt, d.cell_off, post.theta[:, c, k])
+                worst = max(worst, abs(post.log_likelihoods[c, k] / ref - 1))
+        moved += int(np.sum(post.theta[:, :, -1] != subs[s][0]))
+    print("mixed hierarchical: stored-LL max rel %.2e" % worst)
+    assert worst <= 1e-6
+    assert moved > 0
