@@ -1,0 +1,168 @@
+"""Host-side logic on the CPU: design builder, priors, the C-ABI library's
+exports, the batched path's arithmetic (host instantiation) and diagnostics."""
+import ctypes as C
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+import fixtures as fx
+import oracle_lib as O
+from ggdmc_amd import _lib, diagnostics, priors, simulate
+from ggdmc_amd.model import build_model
+from ggdmc_amd.runs import PriorSpec
+
+ROOT = fx.ROOT
+
+
+# ------------------------------------------------------------ C ABI library
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "ggdmc_hip.h")).read()
+    names = set(re.findall(r"^\s*(?:int|const char|void)\s*\*?\s*(ggdmc_\w+)\s*\(", hdr, re.M))
+    assert {"ggdmc_run_subject", "ggdmc_run_hyper", "ggdmc_run", "ggdmc_run_batch",
+            "ggdmc_loglike", "ggdmc_last_error"} <= names
+    lib = C.CDLL(_lib.LIB_PATH)
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_without_gpu_calls_fail_loudly():
+    if _lib.device_count() > 0:
+        pytest.skip("a GPU is present")
+    import ggdmc_amd as g
+    fix = fx.load("lba_data6")
+    with pytest.raises(g.GgdmcError):
+        g.loglike(fix.model, fix.sub_data, fix.sub_theta[:, :2].T)
+
+
+# ------------------------------------------------------------ design builder
+@pytest.mark.parametrize("name", fx.FIXTURES)
+def test_build_model_reproduces_reference_designs(name):
+    fix = fx.load(name)
+    sp = json.loads(str(fix.z["design_spec_json"]))
+    m = build_model(sp["p_map"], sp["match_map"], sp["factors"], sp["constants"],
+                    sp["accumulators"])
+    assert m.cell_names == fix.model.cell_names
+    assert m.pnames == fix.model.pnames
+    assert (m.par_src == fix.model.par_src).all()
+    assert (m.par_const == fix.model.par_const).all()
+    assert (m.winner == fix.model.winner).all()
+
+
+# ------------------------------------------------------------------ priors
+def test_host_priors_match_oracle():
+    fix = fx.load("lba_data6")
+    rng = np.random.default_rng(1)
+    for pr in (fix.sub_prior, fix.pop_prior, fix.hyper_prior):
+        for _ in range(20):
+            x = rng.uniform(-0.5, 11, pr.npar)
+            a = priors.sumlogprior(pr, x)
+            b = O.sumlogprior(fx.oprior(pr), x)
+            assert (a == b) or abs(a - b) < 1e-12 * max(1, abs(b))
+    phi = fix.z["phi_theta"][:, 0]
+    X = fix.z["pop_theta"][:, :, 0]
+    assert abs(priors.sumloghlike(fix.pop_prior, phi, X) / fix.z["phi_ll"][0] - 1) < 1e-13
+
+
+def test_rprior_within_support_and_moments():
+    rng = np.random.default_rng(2)
+    pr = PriorSpec.tnorm([0.5, 2.0], [1.0, 0.3], 0.0, np.inf)
+    x = priors.rprior(pr, 200000, rng)
+    assert (x >= 0).all()
+    from scipy import stats
+    for k in range(2):
+        a = (0 - pr.p0[k]) / pr.p1[k]
+        m = stats.truncnorm.mean(a, np.inf, loc=pr.p0[k], scale=pr.p1[k])
+        assert abs(x[:, k].mean() - m) < 0.01
+
+
+# -------------------------------------- batched-path arithmetic on the host
+_HOST = os.path.join(ROOT, "tests", "native", "liblba_math_host.so")
+
+
+def _mixed(fix, data, theta):
+    L = C.CDLL(_HOST)
+    L.host_mixed_terms.restype = C.c_double
+    d, i32, i64 = C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int64)
+    L.host_mixed_terms.argtypes = [i32, i32, d, C.c_int32, C.c_int32, d, i64, d, d]
+    m = fix.model
+    arrs = [np.ascontiguousarray(a, t) for a, t in ((m.winner, np.int32), (m.par_src, np.int32),
+                                                   (m.par_const, np.float64))]
+    rt = np.ascontiguousarray(data.rt)
+    off = np.ascontiguousarray(data.cell_off, np.int64)
+    th = np.ascontiguousarray(theta, np.float64)
+    return L.host_mixed_terms(arrs[0].ctypes.data_as(i32), arrs[1].ctypes.data_as(i32),
+                              arrs[2].ctypes.data_as(d), m.n_cell, m.n_acc,
+                              rt.ctypes.data_as(d), off.ctypes.data_as(i64),
+                              th.ctypes.data_as(d), None)
+
+
+@pytest.mark.parametrize("name", fx.FIXTURES)
+def test_mixed_arithmetic_on_goldens(name):
+    # the batched path's formulation, with libm transcendentals
+    fix = fx.load(name)
+    e = [abs(_mixed(fix, fix.sub_data, fix.sub_theta[:, k]) / fix.sub_ll[k] - 1)
+         for k in range(fix.sub_theta.shape[1])]
+    for s in range(0, fix.n_subjects, 3):
+        d = fix.pop_data(s)
+        for k in range(fix.z["pop_theta"].shape[2]):
+            e.append(abs(_mixed(fix, d, fix.z["pop_theta"][s, :, k]) / fix.z["pop_ll"][s, k] - 1))
+    e = np.array(e)
+    assert (e <= 1e-6).mean() >= 0.998 and e.max() < 1e-4
+
+
+def test_mixed_arithmetic_near_posterior():
+    fix = fx.load("lba_data6")
+    worst = 0.0
+    for s in range(0, fix.n_subjects, 8):
+        d = fix.pop_data(s)
+        for th in fx.near_thetas(fix, 40, seed=s):
+            ref = O.lba_loglike(fix.omodel, d.rt, d.cell_off, th)
+            worst = max(worst, abs(_mixed(fix, d, th) / ref - 1))
+    assert worst <= 1e-6, worst
+
+
+# -------------------------------------------------------------- simulator
+def test_simulator_rule_recovers_choice_probabilities():
+    # P(response = accumulator 0) = int f_0 S_1 dt from the oracle's density;
+    # the generation rule redraws only when every drift is negative, which
+    # equals the likelihood's truncated drifts when P(drift < 0) is negligible
+    m = build_model(p_map={"A": ["1"], "B": ["1"], "t0": ["1"], "mean_v": ["M"], "sd_v": ["M"],
+                           "st0": ["1"]}, match_map={"M": {"s1": "r1", "s2": "r2"}},
+                    factors={"S": ["s1", "s2"]}, constants={"st0": 0.0, "sd_v.false": 0.3},
+                    accumulators=["r1", "r2"])
+    theta = np.array([0.5, 1.0, 1.0, 2.0, 0.3, 0.2])  # A B mean_v.false mean_v.true sd_v.true t0
+    assert m.pnames == ["A", "B", "mean_v.false", "mean_v.true", "sd_v.true", "t0"]
+    rng = np.random.default_rng(0)
+    data = simulate.simulate_subject(m, theta, 40000, rng)
+    n = np.diff(data.cell_off)
+    # cells: s1.r1 (correct), s1.r2, s2.r1, s2.r2 (correct)
+    p_correct = (n[0] + n[3]) / n.sum()
+    om = O.Model(m.winner, m.par_src, m.par_const, 6)
+    grid = np.linspace(0.2 + 1e-6, 30, 200001)
+    off = np.array([0, len(grid), len(grid), len(grid), len(grid)], np.int64)
+    f = np.exp(O.lba_trial_terms(om, grid, off, theta))
+    p = np.trapezoid(f, grid)
+    assert abs(p_correct - p) < 0.006
+
+
+# ------------------------------------------------------------ diagnostics
+def test_psrf_and_mcse_on_known_processes():
+    rng = np.random.default_rng(0)
+    iid = rng.normal(size=(2, 8, 4000))
+    assert np.all(np.abs(diagnostics.psrf(iid) - 1) < 0.01)
+    assert abs(diagnostics.mpsrf(iid) - 1) < 0.02
+    shifted = iid.copy()
+    shifted[:, :4] += 1.0
+    assert np.all(diagnostics.psrf(shifted) > 1.1)
+    # AR(1) with rho: variance of the mean = sigma^2 / (1 - rho)^2 / n
+    rho, n, m = 0.8, 20000, 4
+    e = rng.normal(size=(m, n))
+    x = np.zeros((m, n))
+    for t in range(1, n):
+        x[:, t] = rho * x[:, t - 1] + e[:, t]
+    se = diagnostics.mcse(x[None, :, :])[0]
+    want = 1 / (1 - rho) / np.sqrt(n * m)
+    assert abs(se / want - 1) < 0.15
