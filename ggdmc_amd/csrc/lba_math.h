@@ -116,7 +116,8 @@ GGDMC_FN double mixed_log(double x) {
         ex -= 1;
     }
     const float mf = (float)m;
-    const double r = (m - (double)mf) / (double)mf; /* log(m) = log(mf) + r */
+    /* log(m) = log(mf) + r with r = (m - mf)/mf, |r| < 2^-24: an FP32 reciprocal suffices */
+    const double r = (m - (double)mf) * (double)Ops::rcp(mf);
     return ((double)Ops::log2(mf) + (double)ex) * kLn2 + r;
 }
 

@@ -435,12 +435,13 @@ __global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs 
 
 /* ------------------------------------------------------------- phi level */
 struct PhiLds {
-    double *ph, *plp, *pll, *st, *prop, *U, *tlp, *red;
+    double *ph, *plp, *pll, *st, *prop, *U, *tlp, *red, *tm;
     int *sub, *flag;
 };
 
 __host__ __device__ inline size_t phi_lds_bytes(int n, int P) {
-    const size_t dbl = (size_t)n * 2 * P + 2 * n + (size_t)n * P * 4 + (size_t)n * 2 * P + 2 * n + kWaves;
+    const size_t dbl = (size_t)n * 2 * P + 2 * n + (size_t)n * P * 4 + (size_t)n * 2 * P + 2 * n +
+                       kWaves + (size_t)2 * n * P;
     return dbl * sizeof(double) + (size_t)(n + 4) * sizeof(int);
 }
 
@@ -455,10 +456,29 @@ __device__ PhiLds phi_carve(char *base, int n, int P) {
     L.U = p; p += n;
     L.tlp = p; p += n;
     L.red = p; p += kWaves;
+    L.tm = p; p += (size_t)2 * n * P;
     int *q = reinterpret_cast<int *>(p);
     L.sub = q; q += n;
     L.flag = q;
     return L;
+}
+
+/* One population parameter's share of sumloghlike from its sufficient
+ * statistics (mean, M2, min, max of the S subject values):
+ *   sum_s log tnorm(x_s; mu, sg, lo, up)
+ *     = -(M2 + S (mean - mu)^2) / (2 sg^2) - S (log sqrt(2 pi) + log sg + log Z)
+ * NaN for sg <= 0 or NaN inputs, -inf when a subject value is outside
+ * [lo, up]; summing the P shares reproduces the per-subject sum's NaN / -inf. */
+__device__ double hll_term(const Dev &d, const JobDev &job, int p, double mu, double sg,
+                           const double *st, int S) {
+    const double mean = st[0], M2 = st[1], mn = st[2], mx = st[3];
+    if (!(sg > 0.0) || mean != mean || mu != mu) return __builtin_nan("");
+    const double lo = d.pr_lo[job.prior_off + p], up = d.pr_up[job.prior_off + p];
+    if (mn < lo || mx > up) return -__builtin_inf();
+    const double Z = tnorm_mass((lo - mu) / sg, (up - mu) / sg);
+    const double dm = mean - mu;
+    return -(M2 + (double)S * dm * dm) / (2.0 * sg * sg) -
+           (double)S * (0.918938533204672741780329736406 + ::log(sg) + ::log(Z));
 }
 
 /* sumloghlike (src/de.cpp:245-270) from per-parameter sufficient statistics
@@ -466,21 +486,9 @@ __device__ PhiLds phi_carve(char *base, int n, int P) {
  *   = -(M2 + S (mean - mu)^2) / (2 sg^2) - S (log sqrt(2 pi) + log sg + log Z). */
 __device__ double hyper_ll(const Dev &d, const JobDev &job, const double *phi, const double *st,
                            int S) {
-    const int P = d.P;
     double out = 0.0;
-    bool outside = false;
-    for (int p = 0; p < P; ++p) {
-        const double mu = phi[p], sg = phi[P + p];
-        const double mean = st[p * 4 + 0], M2 = st[p * 4 + 1], mn = st[p * 4 + 2], mx = st[p * 4 + 3];
-        if (!(sg > 0.0) || mean != mean || mu != mu) return __builtin_nan("");
-        const double lo = d.pr_lo[job.prior_off + p], up = d.pr_up[job.prior_off + p];
-        if (mn < lo || mx > up) outside = true;
-        const double Z = tnorm_mass((lo - mu) / sg, (up - mu) / sg);
-        const double dm = mean - mu;
-        out += -(M2 + (double)S * dm * dm) / (2.0 * sg * sg) -
-               (double)S * (0.918938533204672741780329736406 + ::log(sg) + ::log(Z));
-    }
-    return outside ? -__builtin_inf() : out;
+    for (int p = 0; p < d.P; ++p) out += hll_term(d, job, p, phi[p], phi[d.P + p], st + p * 4, S);
+    return out;
 }
 
 __device__ double hyper_logprior(const Dev &d, const JobDev &job, const double *x) {
@@ -492,53 +500,41 @@ __device__ double hyper_logprior(const Dev &d, const JobDev &job, const double *
     return s;
 }
 
-/* Sufficient statistics of chain c over the run's subjects, two passes. */
-__device__ void chain_stats(const Dev &d, const JobDev &job, int c, double *out, double *red) {
+/* Sufficient statistics of chain c over the run's subjects, two passes;
+ * each group of 16 lanes reduces one parameter with a fixed shuffle tree. */
+__device__ void chain_stats(const Dev &d, const JobDev &job, int c, double *out) {
     const int P = d.P, n = d.nchain, S = job.nsubject;
-    for (int p = 0; p < P; ++p) {
+    const int l16 = threadIdx.x & 15, ngrp = blockDim.x >> 4;
+    for (int p = threadIdx.x >> 4; p < P; p += ngrp) {
         double sum = 0.0, mn = __builtin_inf(), mx = -__builtin_inf();
-        for (int s = threadIdx.x; s < S; s += blockDim.x) {
+        int nan = 0;
+        for (int s = l16; s < S; s += 16) {
             const double x = d.theta[((size_t)(job.first_subject + s) * n + c) * P + p];
             sum += x;
             mn = fmin(mn, x);
             mx = fmax(mx, x);
-            if (x != x) { mn = x; mx = x; }
+            nan |= x != x;
         }
-        const double mean = block_sum(sum, red) / (double)S;
-        /* min / max / NaN flags through the same fixed tree */
-        const double nan_flag = block_sum((mn != mn || mx != mx) ? 1.0 : 0.0, red);
+#pragma unroll
+        for (int off = 8; off > 0; off >>= 1) {
+            sum += __shfl_xor(sum, off, 16);
+            mn = fmin(mn, __shfl_xor(mn, off, 16));
+            mx = fmax(mx, __shfl_xor(mx, off, 16));
+            nan |= __shfl_xor(nan, off, 16);
+        }
+        const double mean = sum / (double)S;
         double m2 = 0.0;
-        for (int s = threadIdx.x; s < S; s += blockDim.x) {
+        for (int s = l16; s < S; s += 16) {
             const double x = d.theta[((size_t)(job.first_subject + s) * n + c) * P + p];
             m2 += (x - mean) * (x - mean);
         }
-        m2 = block_sum(m2, red);
-        /* min and max via wave reductions */
-        double a = mn, b = mx;
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            a = fmin(a, __shfl_xor(a, off, 64));
-            b = fmax(b, __shfl_xor(b, off, 64));
-        }
-        __syncthreads();
-        if ((threadIdx.x & 63) == 0) {
-            red[threadIdx.x >> 6] = a;
-        }
-        __syncthreads();
-        double gmn = __builtin_inf();
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) gmn = fmin(gmn, red[w]);
-        __syncthreads();
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = b;
-        __syncthreads();
-        double gmx = -__builtin_inf();
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) gmx = fmax(gmx, red[w]);
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const double nanv = __builtin_nan("");
-            out[p * 4 + 0] = nan_flag > 0.0 ? nanv : mean;
+        for (int off = 8; off > 0; off >>= 1) m2 += __shfl_xor(m2, off, 16);
+        if (l16 == 0) {
+            out[p * 4 + 0] = nan ? __builtin_nan("") : mean;
             out[p * 4 + 1] = m2;
-            out[p * 4 + 2] = gmn;
-            out[p * 4 + 3] = gmx;
+            out[p * 4 + 2] = mn;
+            out[p * 4 + 3] = mx;
         }
     }
 }
@@ -610,14 +606,13 @@ __device__ void phi_update(const Dev &d, int f, const JobDev &job, uint32_t iter
         const int para = blocked ? sw : -1;
         const Stream st = make_stream(seed, iter + d.philox_offset, unit, (uint32_t)sw);
         for (int g = 0; g < G; ++g) {
-            const int lo = group_lo(g, n, G), hi = group_lo(g + 1, n, G);
+            const int lo = group_lo(g, n, G), hi = group_lo(g + 1, n, G), ng = hi - lo;
+            /* proposals from partners outside the group (src/de.cpp:402-423) */
             for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-                const double *sti = L.st + (size_t)(hier ? i : 0) * P * 4;
-                if (hier) L.pll[i] = hyper_ll(d, job, L.ph + (size_t)i * P2, sti, S);
-                const double cur = L.pll[i] + L.plp[i];
                 int m0, m1;
                 double U;
                 partners(st, i, lo, hi, n, m0, m1, U);
+                L.U[i] = U;
                 double *x = L.prop + (size_t)i * P2;
                 const double *x0 = L.ph + (size_t)m0 * P2, *x1 = L.ph + (size_t)m1 * P2;
                 for (int p = 0; p < P2; ++p) x[p] = L.ph[(size_t)i * P2 + p];
@@ -628,15 +623,31 @@ __device__ void phi_update(const Dev &d, int f, const JobDev &job, uint32_t iter
                         x[p] += jitter(st, (uint32_t)i, p, d.rp) + job.gamma * (x0[p] - x1[p]);
                 }
                 L.tlp[i] = hyper_logprior(d, job, x);
-                L.U[i] = hyper_ll(d, job, x, sti, S); /* proposal log-likelihood */
-                if (!mh_accept(L.tlp[i] + L.U[i], cur, U)) L.tlp[i] = __builtin_nan("");
+            }
+            __syncthreads();
+            /* per (chain, parameter): the refreshed current share (src/de.cpp:397)
+             * and the proposal's share of sumloghlike */
+            const int nt = ng * P;
+            for (int e = threadIdx.x; e < (hier ? 2 : 1) * nt; e += blockDim.x) {
+                const int which = e / nt, r = e - which * nt;
+                const int i = lo + r / P, p = r - (r / P) * P;
+                const double *sti = L.st + (size_t)(hier ? i : 0) * P * 4 + p * 4;
+                const double *v = which == 0 ? L.prop + (size_t)i * P2 : L.ph + (size_t)i * P2;
+                L.tm[(size_t)which * n * P + (size_t)i * P + p] = hll_term(d, job, p, v[p], v[P + p], sti, S);
             }
             __syncthreads();
             for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-                if (L.tlp[i] == L.tlp[i]) {
+                double tll = 0.0, ref = 0.0;
+                for (int p = 0; p < P; ++p) {
+                    tll += L.tm[(size_t)i * P + p];
+                    if (hier) ref += L.tm[(size_t)n * P + (size_t)i * P + p];
+                }
+                if (hier) L.pll[i] = ref;
+                const double cur = L.pll[i] + L.plp[i];
+                if (mh_accept(L.tlp[i] + tll, cur, L.U[i])) {
                     for (int p = 0; p < P2; ++p) L.ph[(size_t)i * P2 + p] = L.prop[(size_t)i * P2 + p];
                     L.plp[i] = L.tlp[i];
-                    L.pll[i] = L.U[i];
+                    L.pll[i] = tll;
                 }
             }
             __syncthreads();
@@ -675,7 +686,7 @@ __global__ __launch_bounds__(kBlock) void phi_kernel(Dev d, uint32_t iter_off) {
         phi_update(d, f, job, iter, true, L);
         return;
     }
-    chain_stats(d, job, c, d.stats + ((size_t)f * n + c) * P * 4, L.red);
+    chain_stats(d, job, c, d.stats + ((size_t)f * n + c) * P * 4);
     __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) {
