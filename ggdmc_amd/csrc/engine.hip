@@ -570,20 +570,23 @@ void ggdmc_session::autotune() {
     const uint32_t it0 = next_iter;
     float best = 1e30f;
     int best_t = 0, best_l = 2;
-    const int reps = 3;
+    /* time graph replays, as production runs them */
+    const int K = 4;
     for (int t : cands) {
         set_tiles(t);
         if (d.n_tiles > 65536) continue;
         for (int l : {1, 2}) {
             tpl = l;
             next_iter = it0;
-            run_batched(1); /* warm-up */
+            capture(K);
+            run_batched(K); /* warm-up */
             HIPCHECK(hipEventRecord(ev0, stream));
-            run_batched(reps);
+            run_batched(2 * K);
             HIPCHECK(hipEventRecord(ev1, stream));
             HIPCHECK(hipEventSynchronize(ev1));
             float ms = 0.f;
             HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
+            drop_graph();
             restore();
             if (ms < best) {
                 best = ms;
