@@ -45,12 +45,28 @@ def parse():
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--workload", default="c3")
-    ap.add_argument("--plain-steps", type=int, default=20)
+    ap.add_argument("--plain-steps", type=int, default=0,
+                    help="iterations of the plain FP64 path (0: 20, or 3 for c4 / c5)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--profile-iters", type=int, default=20)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 256 MiB device-to-device copy (counter calibration)")
     return ap.parse_args()
+
+
+def calibrate_copy():
+    """A plain copy of known size: FETCH_SIZE / WRITE_SIZE of this kernel
+    scale the counters of the sampler's kernels to bytes."""
+    import torch
+    n = 64 * 1024 * 1024   # 256 MiB of float32
+    x = torch.ones(n, device="cuda", dtype=torch.float32)
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
 
 
 def cpu_baseline(w, budget_s):
@@ -91,6 +107,9 @@ def cpu_baseline(w, budget_s):
 
 def main():
     a = parse()
+    if a.calibrate_copy:
+        calibrate_copy()
+        return
     if a.gpus != 1:
         print("bench.py measures one GPU: this build runs no multi-GPU configuration "
               "(--gpus %d requested)" % a.gpus, file=sys.stderr)
@@ -139,28 +158,32 @@ def main():
     res = sess.download()
     sess.close()
 
-    # per-proposal |dLL|: every stored subject state against the FP64 path
+    # per-proposal |dLL|: stored subject states against the FP64 path (every
+    # subject for up to 64 subjects, an even subsample beyond)
     worst = 0.0
     n_cmp = 0
+    pairs = []
     for job, r in zip(w.jobs, res):
         datas = [job.subject] if job.kind == g.KIND_SUBJECT else job.subjects
-        for d, post in zip(datas, r["subjects"]):
-            k = post.nmc - 1
-            th = post.theta[:, :, k].T
-            ref = g.loglike(w.model, d, th, g.FP64_PLAIN)
-            worst = max(worst, float(np.max(np.abs(post.log_likelihoods[:, k] / ref - 1))))
-            n_cmp += len(ref)
+        pairs.extend(zip(datas, r["subjects"]))
+    step = max(1, len(pairs) // 64)
+    for d, post in pairs[::step]:
+        k = post.nmc - 1
+        ref = g.loglike(w.model, d, post.theta[:, :, k].T, g.FP64_PLAIN)
+        worst = max(worst, float(np.max(np.abs(post.log_likelihoods[:, k] / ref - 1))))
+        n_cmp += len(ref)
 
     plain = None
     if not a.no_plain:
-        pcfg = w.config(nmc=a.plain_steps + 3, precision=g.FP64_PLAIN)
+        n_plain = a.plain_steps or (3 if w.name in ("c4", "c5") else 20)
+        pcfg = w.config(nmc=n_plain + 3, precision=g.FP64_PLAIN)
         ps = g.Session(pcfg, w.model, w.jobs)
         ps.advance(2)
-        pms = ps.advance(a.plain_steps)
+        pms = ps.advance(n_plain)
         ps.close()
-        plain = a.plain_steps / (pms / 1e3)
+        plain = n_plain / (pms / 1e3)
         print("[bench] plain FP64: %d iterations in %.1f ms: %.1f iter/s"
-              % (a.plain_steps, pms, plain), file=sys.stderr, flush=True)
+              % (n_plain, pms, plain), file=sys.stderr, flush=True)
 
     cpu = None
     if not a.no_cpu:

@@ -38,6 +38,7 @@ struct Dev {
     double rp, sub_mig, pop_mig;
     int32_t S, n_tiles, n_phi, n_jobs;
     uint32_t philox_offset;     /* Philox iteration = local iteration + this */
+    uint32_t last_iter;         /* last local iteration of the run (nmc*thin - 1) */
 
     const int32_t *par_src;     /* [n_cell][n_acc][6] */
     const double *par_const;
@@ -64,7 +65,7 @@ struct Dev {
     double *stats;              /* [F][n][P][4]: mean, M2, min, max over subjects */
 
     double *partial;
-    uint32_t *counter;          /* [S] tile arrival counters, then [F] */
+    uint32_t *counter;          /* [S] tile arrivals, [F] stats arrivals, [jobs] subject arrivals */
     unsigned long long *evals;  /* trial-likelihood evaluations */
     const uint32_t *iter_base;  /* device iteration base (graph replays) */
 
@@ -79,6 +80,8 @@ struct SweepArgs {
     int32_t g_begin, g_end;
     int32_t final_launch; /* last subject launch of the iteration: store if due */
     int32_t chain_batch;  /* chains whose constants fit in LDS at once */
+    int32_t fuse_phi;     /* final launch: the last subject of a hierarchical run
+                             performs the next iteration's population step */
 };
 
 } // namespace ggdmc

@@ -99,13 +99,6 @@ int device_lds_limit() {
     return (int)std::min<size_t>(prop.sharedMemPerBlock, 65536);
 }
 
-int device_cus() {
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess)
-        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    return cus;
-}
-
 } // namespace
 
 /* ================================================================ session */
@@ -123,6 +116,8 @@ struct ggdmc_session {
     size_t sweep_lds = 0, phi_lds = 0;
     int nsweep_sub = 1;
     bool fused_groups = true;
+    bool fuse_phi = false;       /* population step fused into the final subject launch */
+    uint32_t phi_ready = 0;      /* iteration whose population step has been done */
     uint32_t next_iter = 1;      /* next local iteration to run (1-based) */
     uint32_t last_iter = 0;      /* nmc*thin - 1 */
     uint32_t *iter_base = nullptr;
@@ -149,7 +144,8 @@ struct ggdmc_session {
 
     void build(const ggdmc_config *c, const ggdmc_model *model, const ggdmc_job *js, int nj);
     void set_tiles(int tpt_new);
-    void enqueue_iteration(uint32_t off, bool graph_capture);
+    void enqueue_iteration(uint32_t off, std::vector<hipEvent_t> *ev = nullptr);
+    void ensure_phi_ready();
     void run_batched(uint32_t n_iter);
     void run_plain(uint32_t n_iter);
     void autotune();
@@ -363,8 +359,8 @@ void ggdmc_session::build(const ggdmc_config *c, const ggdmc_model *model, const
     d.phi_lp = pool.upload(plp0, stream);
     d.phi_ll = pool.upload(pll0, stream);
     d.stats = pool.alloc<double>((size_t)std::max(F, 1) * n * P * 4);
-    d.counter = pool.alloc<uint32_t>((size_t)S + F + 1);
-    HIPCHECK(hipMemsetAsync(d.counter, 0, sizeof(uint32_t) * (S + F + 1), stream));
+    d.counter = pool.alloc<uint32_t>((size_t)S + F + nj + 1);
+    HIPCHECK(hipMemsetAsync(d.counter, 0, sizeof(uint32_t) * (S + F + nj + 1), stream));
     d.evals = pool.alloc<unsigned long long>(1);
     HIPCHECK(hipMemsetAsync(d.evals, 0, sizeof(unsigned long long), stream));
     iter_base = pool.alloc<uint32_t>(1);
@@ -404,7 +400,20 @@ void ggdmc_session::build(const ggdmc_config *c, const ggdmc_model *model, const
 
     nsweep_sub = cfg.is_pblocked ? P : 1;
     last_iter = (uint32_t)(cfg.nmc * cfg.thin - 1);
+    d.last_iter = last_iter;
     next_iter = 1;
+    /* fuse the population step into the final subject launch when one
+     * workgroup reduces the statistics quickly (small runs), no hyper-only run
+     * needs the standalone population kernel, and the path is batched */
+    {
+        bool any_hyper = false;
+        int64_t work = 0;
+        for (const auto &j : jobs) {
+            any_hyper |= j.kind == GGDMC_KIND_HYPER;
+            if (j.kind == GGDMC_KIND_HIER) work = std::max<int64_t>(work, (int64_t)j.nsubject * n * P);
+        }
+        fuse_phi = mode != GGDMC_FP64_PLAIN && F > 0 && !any_hyper && work <= 65536;
+    }
 
     /* LDS budget: chains whose AccK tables are resident at once */
     const int ng = (n + G - 1) / G + 1;
@@ -417,6 +426,7 @@ void ggdmc_session::build(const ggdmc_config *c, const ggdmc_model *model, const
     require(sweep_lds <= (size_t)limit, "design too large for LDS");
     phi_lds = ggdmc_internal_phi_lds(n, P);
     require(phi_lds <= (size_t)limit, "population state too large for LDS");
+    if (fuse_phi) sweep_lds = std::max(sweep_lds, phi_lds);
 
     tpl = 2;
     set_tiles(cfg.trials_per_tile > 0 ? cfg.trials_per_tile : 0);
@@ -456,21 +466,41 @@ void ggdmc_session::set_tiles(int tpt_new) {
     drop_graph();
 }
 
-void ggdmc_session::enqueue_iteration(uint32_t off, bool) {
-    if (d.n_phi > 0) HIPCHECK(ggdmc_internal_launch_phi(&d, off, phi_lds, stream));
+/* One iteration's launches. With fuse_phi the population step of this
+ * iteration was done by the previous iteration's final launch (or by
+ * ensure_phi_ready), and this iteration's final launch does the next one.
+ * ev, if given, receives an event pair around every subject launch. */
+void ggdmc_session::enqueue_iteration(uint32_t off, std::vector<hipEvent_t> *ev) {
+    if (d.n_phi > 0 && !fuse_phi) HIPCHECK(ggdmc_internal_launch_phi(&d, off, phi_lds, stream));
     if (d.S == 0) return;
     const int lk = mode == GGDMC_FP64_BATCHED ? GGDMC_FP64_BATCHED : GGDMC_MIXED_BATCHED;
+    const int ngl = fused_groups ? 1 : d.G;
     for (int sw = 0; sw < nsweep_sub; ++sw) {
-        if (fused_groups) {
-            SweepArgs a{off, sw, 0, d.G, sw == nsweep_sub - 1, chain_batch};
-            HIPCHECK(ggdmc_internal_launch_sweep(&d, &a, lk, tpl, sweep_lds, stream));
-        } else {
-            for (int g = 0; g < d.G; ++g) {
-                SweepArgs a{off, sw, g, g + 1, (sw == nsweep_sub - 1 && g == d.G - 1), chain_batch};
-                HIPCHECK(ggdmc_internal_launch_sweep(&d, &a, lk, tpl, sweep_lds, stream));
+        for (int g = 0; g < ngl; ++g) {
+            const int fin = sw == nsweep_sub - 1 && g == ngl - 1;
+            SweepArgs a{off, sw, fused_groups ? 0 : g, fused_groups ? d.G : g + 1, fin, chain_batch,
+                        (int)(fin && fuse_phi)};
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (ev) {
+                HIPCHECK(hipEventCreate(&e0));
+                HIPCHECK(hipEventCreate(&e1));
+                ev->push_back(e0);
+                ev->push_back(e1);
+                HIPCHECK(hipEventRecord(e0, stream));
             }
+            HIPCHECK(ggdmc_internal_launch_sweep(&d, &a, lk, tpl, sweep_lds, stream));
+            if (ev) HIPCHECK(hipEventRecord(e1, stream));
         }
     }
+}
+
+/* With fuse_phi: runs the population step of next_iter if no earlier launch did. */
+void ggdmc_session::ensure_phi_ready() {
+    if (!fuse_phi || phi_ready == next_iter) return;
+    set_base_kernel<<<1, 1, 0, stream>>>(iter_base, next_iter);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(ggdmc_internal_launch_phi(&d, 0, phi_lds, stream));
+    phi_ready = next_iter;
 }
 
 void ggdmc_session::capture(int K) {
@@ -479,7 +509,7 @@ void ggdmc_session::capture(int K) {
     hipGraph_t g = nullptr;
     HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     try {
-        for (int k = 0; k < K; ++k) enqueue_iteration((uint32_t)k, true);
+        for (int k = 0; k < K; ++k) enqueue_iteration((uint32_t)k);
     } catch (...) {
         (void)hipStreamEndCapture(stream, &g);
         if (g) (void)hipGraphDestroy(g);
@@ -493,6 +523,8 @@ void ggdmc_session::capture(int K) {
 }
 
 void ggdmc_session::run_batched(uint32_t n_iter) {
+    if (n_iter == 0) return;
+    ensure_phi_ready();
     while (n_iter > 0) {
         if (graph && n_iter >= (uint32_t)graph_iters) {
             set_base_kernel<<<1, 1, 0, stream>>>(iter_base, next_iter);
@@ -503,11 +535,12 @@ void ggdmc_session::run_batched(uint32_t n_iter) {
         } else {
             set_base_kernel<<<1, 1, 0, stream>>>(iter_base, next_iter);
             HIPCHECK(hipGetLastError());
-            enqueue_iteration(0, false);
+            enqueue_iteration(0);
             next_iter += 1;
             n_iter -= 1;
         }
     }
+    phi_ready = next_iter;   /* fused: the last launch did the next population step */
 }
 
 void ggdmc_session::run_plain(uint32_t n_iter) {
@@ -568,6 +601,7 @@ void ggdmc_session::autotune() {
     };
     save();
     const uint32_t it0 = next_iter;
+    const uint32_t ready0 = phi_ready;
     float best = 1e30f;
     int best_t = 0, best_l = 2;
     /* time graph replays, as production runs them */
@@ -588,6 +622,7 @@ void ggdmc_session::autotune() {
             HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
             drop_graph();
             restore();
+            phi_ready = ready0;
             if (ms < best) {
                 best = ms;
                 best_t = t;
@@ -709,27 +744,16 @@ int ggdmc_session_profile(ggdmc_session *s, int32_t n_iter, double *subject_kern
         }
         const uint32_t left = s->next_iter <= s->last_iter ? s->last_iter - s->next_iter + 1 : 0;
         const uint32_t n = std::min<uint32_t>((uint32_t)n_iter, left);
-        const int per_iter = s->nsweep_sub * (s->fused_groups ? 1 : s->d.G);
-        std::vector<hipEvent_t> ev(2 * (size_t)n * per_iter);
-        for (auto &e : ev) HIPCHECK(hipEventCreate(&e));
-        const int lk = s->mode == GGDMC_FP64_BATCHED ? GGDMC_FP64_BATCHED : GGDMC_MIXED_BATCHED;
-        size_t q = 0;
+        std::vector<hipEvent_t> ev;
+        s->ensure_phi_ready();
         for (uint32_t k = 0; k < n; ++k) {
             set_base_kernel<<<1, 1, 0, s->stream>>>(s->iter_base, s->next_iter);
             HIPCHECK(hipGetLastError());
-            if (s->d.n_phi > 0) HIPCHECK(ggdmc_internal_launch_phi(&s->d, 0, s->phi_lds, s->stream));
-            for (int sw = 0; sw < s->nsweep_sub; ++sw) {
-                const int ng = s->fused_groups ? 1 : s->d.G;
-                for (int g = 0; g < ng; ++g) {
-                    SweepArgs a{0, sw, s->fused_groups ? 0 : g, s->fused_groups ? s->d.G : g + 1,
-                                (sw == s->nsweep_sub - 1 && g == ng - 1), s->chain_batch};
-                    HIPCHECK(hipEventRecord(ev[q++], s->stream));
-                    HIPCHECK(ggdmc_internal_launch_sweep(&s->d, &a, lk, s->tpl, s->sweep_lds, s->stream));
-                    HIPCHECK(hipEventRecord(ev[q++], s->stream));
-                }
-            }
+            s->enqueue_iteration(0, &ev);
             s->next_iter += 1;
         }
+        s->phi_ready = s->next_iter;
+        const size_t q = ev.size();
         HIPCHECK(hipStreamSynchronize(s->stream));
         double tot = 0.0;
         for (size_t i = 0; i < q; i += 2) {

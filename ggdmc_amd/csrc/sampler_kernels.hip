@@ -312,6 +312,9 @@ __device__ void subject_migration(const Dev &d, const SweepLds &L, const JobDev 
 }
 
 /* ------------------------------------------------------------ sweep kernel */
+__device__ void fused_phi_tail(const Dev &d, int s, const JobDev &job, uint32_t iter,
+                               char *lds_raw);
+
 template <int MODE, int TPL>
 __global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs a) {
     extern __shared__ __attribute__((aligned(16))) char lds_raw[];
@@ -348,7 +351,10 @@ __global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs 
             const int para = (blocked && !single) ? a.sweep : -1;
             subject_migration<MODE, TPL>(d, L, job, s, sd, para, a.chain_batch);
         }
-        if (a.final_launch) store_subject(d, L, s, iter);
+        if (a.final_launch) {
+            store_subject(d, L, s, iter);
+            if (a.fuse_phi && job.kind == GGDMC_KIND_HIER) fused_phi_tail(d, s, job, iter, lds_raw);
+        }
         return;
     }
 
@@ -430,7 +436,10 @@ __global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs 
         }
         __syncthreads();
     }
-    if (a.final_launch) store_subject(d, L, s, iter);
+    if (a.final_launch) {
+        store_subject(d, L, s, iter);
+        if (a.fuse_phi && job.kind == GGDMC_KIND_HIER) fused_phi_tail(d, s, job, iter, lds_raw);
+    }
 }
 
 /* ------------------------------------------------------------- phi level */
@@ -502,10 +511,11 @@ __device__ double hyper_logprior(const Dev &d, const JobDev &job, const double *
 
 /* Sufficient statistics of chain c over the run's subjects, two passes;
  * each group of 16 lanes reduces one parameter with a fixed shuffle tree. */
-__device__ void chain_stats(const Dev &d, const JobDev &job, int c, double *out) {
+__device__ void chain_stats_range(const Dev &d, const JobDev &job, int c0, int c1, double *out) {
     const int P = d.P, n = d.nchain, S = job.nsubject;
     const int l16 = threadIdx.x & 15, ngrp = blockDim.x >> 4;
-    for (int p = threadIdx.x >> 4; p < P; p += ngrp) {
+    for (int q = threadIdx.x >> 4; q < (c1 - c0) * P; q += ngrp) {
+        const int c = c0 + q / P, p = q - (q / P) * P;
         double sum = 0.0, mn = __builtin_inf(), mx = -__builtin_inf();
         int nan = 0;
         for (int s = l16; s < S; s += 16) {
@@ -531,12 +541,17 @@ __device__ void chain_stats(const Dev &d, const JobDev &job, int c, double *out)
 #pragma unroll
         for (int off = 8; off > 0; off >>= 1) m2 += __shfl_xor(m2, off, 16);
         if (l16 == 0) {
-            out[p * 4 + 0] = nan ? __builtin_nan("") : mean;
-            out[p * 4 + 1] = m2;
-            out[p * 4 + 2] = mn;
-            out[p * 4 + 3] = mx;
+            double *o = out + ((size_t)(c - c0) * P + p) * 4;
+            o[0] = nan ? __builtin_nan("") : mean;
+            o[1] = m2;
+            o[2] = mn;
+            o[3] = mx;
         }
     }
+}
+
+__device__ void chain_stats(const Dev &d, const JobDev &job, int c, double *out) {
+    chain_stats_range(d, job, c, c + 1, out);
 }
 
 /* Population step of one run in one workgroup: phi state in LDS, chain i
@@ -698,6 +713,37 @@ __global__ __launch_bounds__(kBlock) void phi_kernel(Dev d, uint32_t iter_off) {
     __threadfence();
     if (threadIdx.x == 0) d.counter[d.S + f] = 0u;
     phi_update(d, f, job, iter, true, L);
+}
+
+/* Called by the workgroup that finished subject s's iteration: the last
+ * subject of the run to finish computes every chain's statistics and the
+ * population step of the next iteration (the run's subjects are all done, so
+ * no workgroup still reads the population state). */
+__device__ void fused_phi_tail(const Dev &d, int s, const JobDev &job, uint32_t iter,
+                               char *lds_raw) {
+    __threadfence();
+    __syncthreads();
+    int *flag = reinterpret_cast<int *>(lds_raw);
+    const int jidx = d.subj_job[s];
+    uint32_t *ctr = d.counter + d.S + d.n_phi + jidx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t old = atomicAdd(ctr, 1u);
+        flag[0] = old == (uint32_t)(job.nsubject - 1);
+    }
+    __syncthreads();
+    const bool last = flag[0] != 0;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    if (threadIdx.x == 0) *ctr = 0u;
+    if (iter + 1 > d.last_iter) return;
+    const int f = job.phi_index, n = d.nchain, P = d.P;
+    chain_stats_range(d, job, 0, n, d.stats + (size_t)f * n * P * 4);
+    __threadfence();
+    __syncthreads();
+    const PhiLds L = phi_carve(lds_raw, n, P);
+    phi_update(d, f, job, iter + 1, true, L);
 }
 
 /* ------------------------------------------------------- plain FP64 path */
