@@ -872,45 +872,77 @@ int ggdmc_loglike(const ggdmc_model *model, const ggdmc_subject *data, const dou
         g_error = "no gfx950 device visible";
         return GGDMC_ENODEV;
     }
-    if (n < 1 || !theta || !ll_out) {
-        g_error = "theta / ll_out missing";
-        return GGDMC_EINVAL;
-    }
+    void *dbuf = nullptr;
     try {
+        require(n >= 1 && theta && ll_out, "theta / ll_out missing");
         require(precision == GGDMC_FP64_PLAIN || precision == GGDMC_MIXED_BATCHED,
                 "loglike precision must be FP64_PLAIN or MIXED_BATCHED");
-        /* one-subject session without sampling: reuse the flattening */
-        std::vector<double> zeros((size_t)model->npar * 3, 0.0), zl(3, 0.0);
-        std::vector<int32_t> dist(model->npar, GGDMC_DIST_UNIF);
-        std::vector<double> p0(model->npar, 0.0), p1(model->npar, 1.0);
-        ggdmc_prior pr{model->npar, dist.data(), p0.data(), p1.data(), p0.data(), p1.data()};
-        ggdmc_state st{zeros.data(), zl.data(), zl.data()};
-        std::vector<double> ot((size_t)model->npar * 3), ol(3), oll(3);
-        ggdmc_samples so{ot.data(), ol.data(), oll.data()};
-        ggdmc_job j{};
-        j.kind = GGDMC_KIND_SUBJECT;
-        j.p_prior = &pr;
-        j.nsubject = 1;
-        j.subjects = data;
-        j.subject_starts = &st;
-        j.subject_outs = &so;
-        ggdmc_config c{};
-        c.nmc = 1;
-        c.thin = 1;
-        c.nchain = 3;
-        c.sweep_groups = 3;
-        c.gamma_precursor = 2.38;
-        c.precision = GGDMC_MIXED_BATCHED;
-        auto s = std::make_unique<ggdmc_session>();
-        s->build(&c, model, &j, 1);
-        double *dth = s->pool.alloc<double>((size_t)n * model->npar);
-        double *dout = s->pool.alloc<double>((size_t)n);
-        HIPCHECK(hipMemcpyAsync(dth, theta, sizeof(double) * n * model->npar, hipMemcpyHostToDevice, s->stream));
-        const size_t lds = ggdmc_internal_sweep_lds(1, model->npar, 1, model->n_cell, model->n_acc, 1);
-        HIPCHECK(ggdmc_internal_launch_loglike(&s->d, dth, n, 0, precision, lds, s->stream, dout));
-        HIPCHECK(hipMemcpyAsync(ll_out, dout, sizeof(double) * n, hipMemcpyDeviceToHost, s->stream));
-        HIPCHECK(hipStreamSynchronize(s->stream));
+        require(model && model->winner && model->par_src && model->par_const, "model missing");
+        require(model->n_cell >= 1 && model->n_cell <= 256 && model->n_acc >= 1 &&
+                    model->n_cell * model->n_acc <= 256 && model->npar >= 1 && model->npar <= 64,
+                "design out of range");
+        require(data && data->cell_off && (data->rt || data->ntrial == 0), "subject data missing");
+        require(data->cell_off[0] == 0 && data->cell_off[model->n_cell] == data->ntrial,
+                "cell_off must start at 0 and end at ntrial");
+        const int nc = model->n_cell, na = model->n_acc, P = model->npar;
+        for (int e = 0; e < nc * na; ++e) {
+            require(model->par_src[e * 6 + 4] == -1 && model->par_const[e * 6 + 4] == 0.0,
+                    "st0 must be the constant 0");
+            for (int f = 0; f < 6; ++f)
+                require(model->par_src[e * 6 + f] >= -1 && model->par_src[e * 6 + f] < P,
+                        "par_src out of range");
+        }
+        for (int c = 0; c < nc; ++c) {
+            require(model->winner[c] >= 0 && model->winner[c] < na, "winner out of range");
+            require(data->cell_off[c + 1] >= data->cell_off[c], "cell_off must be non-decreasing");
+        }
+        /* one packed device buffer: design, trials, thetas, output */
+        const int64_t T = data->ntrial;
+        std::vector<uint16_t> tcell((size_t)T);
+        for (int c = 0; c < nc; ++c)
+            for (int64_t i = data->cell_off[c]; i < data->cell_off[c + 1]; ++i) tcell[i] = (uint16_t)c;
+        const int64_t soff[2] = {0, T};
+        auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+        const size_t o_src = 0, o_const = o_src + al(sizeof(int32_t) * nc * na * 6);
+        const size_t o_win = o_const + al(sizeof(double) * nc * na * 6);
+        const size_t o_rt = o_win + al(sizeof(int32_t) * nc);
+        const size_t o_cell = o_rt + al(sizeof(double) * (size_t)T);
+        const size_t o_off = o_cell + al(sizeof(uint16_t) * (size_t)T);
+        const size_t o_th = o_off + al(sizeof(soff));
+        const size_t o_out = o_th + al(sizeof(double) * (size_t)n * P);
+        const size_t total = o_out + al(sizeof(double) * (size_t)n);
+        std::vector<char> h(total);
+        std::memcpy(&h[o_src], model->par_src, sizeof(int32_t) * nc * na * 6);
+        std::memcpy(&h[o_const], model->par_const, sizeof(double) * nc * na * 6);
+        std::memcpy(&h[o_win], model->winner, sizeof(int32_t) * nc);
+        if (T) std::memcpy(&h[o_rt], data->rt, sizeof(double) * (size_t)T);
+        if (T) std::memcpy(&h[o_cell], tcell.data(), sizeof(uint16_t) * (size_t)T);
+        std::memcpy(&h[o_off], soff, sizeof(soff));
+        std::memcpy(&h[o_th], theta, sizeof(double) * (size_t)n * P);
+        HIPCHECK(hipMalloc(&dbuf, total));
+        char *b = static_cast<char *>(dbuf);
+        HIPCHECK(hipMemcpy(b, h.data(), o_out, hipMemcpyHostToDevice));
+        Dev d{};
+        d.P = P;
+        d.n_acc = na;
+        d.n_cell = nc;
+        d.nchain = 1;
+        d.par_src = reinterpret_cast<const int32_t *>(b + o_src);
+        d.par_const = reinterpret_cast<const double *>(b + o_const);
+        d.winner = reinterpret_cast<const int32_t *>(b + o_win);
+        d.rt = reinterpret_cast<const double *>(b + o_rt);
+        d.tcell = reinterpret_cast<const uint16_t *>(b + o_cell);
+        d.subj_off = reinterpret_cast<const int64_t *>(b + o_off);
+        d.S = 1;
+        const size_t lds = ggdmc_internal_sweep_lds(1, P, 1, nc, na, 1);
+        HIPCHECK(ggdmc_internal_launch_loglike(&d, reinterpret_cast<const double *>(b + o_th), n, 0,
+                                               precision, lds, nullptr,
+                                               reinterpret_cast<double *>(b + o_out)));
+        HIPCHECK(hipMemcpy(ll_out, b + o_out, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost));
+        HIPCHECK(hipFree(dbuf));
+        dbuf = nullptr;
     } catch (const std::exception &e) {
+        if (dbuf) (void)hipFree(dbuf);
         return fail(e);
     }
     return GGDMC_OK;
