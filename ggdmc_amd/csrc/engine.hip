@@ -675,7 +675,8 @@ int ggdmc_session_create(const ggdmc_config *cfg, const ggdmc_model *model, cons
     try {
         s->build(cfg, model, jobs, njobs);
         if (s->mode != GGDMC_FP64_PLAIN) {
-            if (cfg->trials_per_tile == 0) s->autotune();
+            /* tune the tile shape when the run is long enough to repay it */
+            if (cfg->trials_per_tile == 0 && s->last_iter >= 64) s->autotune();
             const uint32_t total = s->last_iter;
             const int K = (int)std::min<uint32_t>(64u, total);
             s->capture(K);
