@@ -212,10 +212,12 @@ GGDMC_FN bool make_acck(const double *theta, const int32_t *src6, const double *
     k.t0 = f[5];
     k.invA = 1.0 / k.A;
     const double x = k.v / k.s;
-    const double Pv = 0.5 * Ops::erfc(-x * kSqrt1_2);
+    /* one erfc gives both tails: q = Phi(-|x|) exactly, the other is 1 - q */
+    const double q = 0.5 * Ops::erfc(std::fabs(x) * kSqrt1_2);
+    const double Pv = x >= 0.0 ? 1.0 - q : (x == x ? q : x);   /* Phi(v/s)  */
+    k.Pc = x >= 0.0 ? q : (x == x ? 1.0 - q : x);              /* Phi(-v/s) */
     k.den = Pv > kFloor ? Pv : (Pv == Pv ? kFloor : Pv);
     k.inv_den = 1.0 / k.den;
-    k.Pc = 0.5 * Ops::erfc(x * kSqrt1_2);
     return f[4] == 0.0;
 }
 
