@@ -4,7 +4,9 @@
  *
  * HBM layout (all arrays device-resident for the whole run):
  *   rt      double [T]          trials of every subject, subject-major, cell-sorted
- *   tcell   uint16 [T]          cell of each trial
+ *   tcode   uint16 [T]          condition << 3 | responding accumulator of each trial
+ *                               (a condition = cells that differ only in the response:
+ *                               every accumulator's parameters are the same)
  *   theta   double [S][n][P]    current subject states (chain-major, P contiguous)
  *   lp, ll  double [S][n]
  *   phi     double [F][n][2P]   population states of hierarchical / hyper runs
@@ -33,19 +35,18 @@ struct JobDev {
 };
 
 struct Dev {
-    int32_t P, n_acc, n_cell, nchain, G, nmc, thin;
+    int32_t P, n_acc, n_cond, nchain, G, nmc, thin;
     int32_t blocked_sub, blocked_pop;
     double rp, sub_mig, pop_mig;
     int32_t S, n_tiles, n_phi, n_jobs;
     uint32_t philox_offset;     /* Philox iteration = local iteration + this */
     uint32_t last_iter;         /* last local iteration of the run (nmc*thin - 1) */
 
-    const int32_t *par_src;     /* [n_cell][n_acc][6] */
+    const int32_t *par_src;     /* [n_cond][n_acc][6] */
     const double *par_const;
-    const int32_t *winner;      /* [n_cell] */
 
     const double *rt;
-    const uint16_t *tcell;
+    const uint16_t *tcode;
     const int64_t *subj_off;    /* [S+1] trial offsets */
     const int32_t *subj_job;    /* [S] */
     const int32_t *subj_unit;   /* [S] Philox unit (index within its run) */

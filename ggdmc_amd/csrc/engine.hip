@@ -157,6 +157,38 @@ struct ggdmc_session {
     }
 };
 
+/* Conditions of a design: cells whose per-accumulator parameter rows are
+ * identical (in an LBA design they differ only in the response). Trials then
+ * carry (condition, responding accumulator) and the per-proposal constants
+ * are built once per (condition, accumulator). */
+struct Condensed {
+    int n_cond = 0;
+    std::vector<int> cond_of_cell;
+    std::vector<int32_t> src;
+    std::vector<double> cst;
+};
+
+static Condensed condense(const ggdmc_model *m) {
+    Condensed r;
+    const int row = m->n_acc * 6;
+    for (int c = 0; c < m->n_cell; ++c) {
+        const int32_t *s = m->par_src + (size_t)c * row;
+        const double *k = m->par_const + (size_t)c * row;
+        int found = -1;
+        for (int q = 0; q < r.n_cond && found < 0; ++q)
+            if (std::equal(s, s + row, r.src.begin() + (size_t)q * row) &&
+                std::equal(k, k + row, r.cst.begin() + (size_t)q * row))
+                found = q;
+        if (found < 0) {
+            found = r.n_cond++;
+            r.src.insert(r.src.end(), s, s + row);
+            r.cst.insert(r.cst.end(), k, k + row);
+        }
+        r.cond_of_cell.push_back(found);
+    }
+    return r;
+}
+
 static void check_prior(const ggdmc_prior *p, int npar, const char *what) {
     require(p != nullptr, std::string(what) + " is null");
     require(p->npar == npar, std::string(what) + ": npar mismatch");
@@ -193,8 +225,9 @@ void ggdmc_session::build(const ggdmc_config *c, const ggdmc_model *model, const
     if (any_lba) {
         require(model != nullptr && model->winner && model->par_src && model->par_const, "model missing");
         require(model->n_acc >= 1 && model->n_cell >= 1 && model->npar >= 1, "empty model");
-        require(model->n_cell <= 256, "at most 256 cells");
-        require(model->n_cell * model->n_acc <= 256 && model->npar <= 64, "design too large");
+        require(model->n_acc <= 8, "at most 8 accumulators");
+        require(model->n_cell <= 4096, "at most 4096 cells");
+        require(model->npar <= 64, "at most 64 parameters");
         P = model->npar;
         for (int e = 0; e < model->n_cell * model->n_acc; ++e) {
             for (int f = 0; f < 6; ++f) {
@@ -222,7 +255,12 @@ void ggdmc_session::build(const ggdmc_config *c, const ggdmc_model *model, const
 
     /* ---- host-side flattening */
     std::vector<double> rt;
-    std::vector<uint16_t> tcell;
+    std::vector<uint16_t> tcode;
+    Condensed cond;
+    if (any_lba) {
+        cond = condense(model);
+        require(cond.n_cond * model->n_acc <= 256, "design has more than 256 condition x accumulator pairs");
+    }
     std::vector<int64_t> subj_off{0};
     std::vector<int32_t> subj_job, subj_unit;
     std::vector<JobDev> jd;
@@ -257,7 +295,7 @@ void ggdmc_session::build(const ggdmc_config *c, const ggdmc_model *model, const
             require(sb->cell_off[cidx + 1] >= sb->cell_off[cidx], "cell_off must be non-decreasing");
             for (int64_t i = sb->cell_off[cidx]; i < sb->cell_off[cidx + 1]; ++i) {
                 rt.push_back(sb->rt[i]);
-                tcell.push_back((uint16_t)cidx);
+                tcode.push_back((uint16_t)((cond.cond_of_cell[cidx] << 3) | model->winner[cidx]));
             }
         }
         subj_off.push_back((int64_t)rt.size());
@@ -320,7 +358,7 @@ void ggdmc_session::build(const ggdmc_config *c, const ggdmc_model *model, const
     const int F = (int)phi_job.size();
     d.P = P;
     d.n_acc = any_lba ? model->n_acc : 1;
-    d.n_cell = any_lba ? model->n_cell : 1;
+    d.n_cond = any_lba ? cond.n_cond : 1;
     d.nchain = n;
     d.G = G;
     d.nmc = cfg.nmc;
@@ -336,12 +374,11 @@ void ggdmc_session::build(const ggdmc_config *c, const ggdmc_model *model, const
     d.philox_offset = (uint32_t)cfg.iter_offset;
 
     if (any_lba) {
-        d.par_src = pool.upload(std::vector<int32_t>(model->par_src, model->par_src + model->n_cell * model->n_acc * 6), stream);
-        d.par_const = pool.upload(std::vector<double>(model->par_const, model->par_const + model->n_cell * model->n_acc * 6), stream);
-        d.winner = pool.upload(std::vector<int32_t>(model->winner, model->winner + model->n_cell), stream);
+        d.par_src = pool.upload(cond.src, stream);
+        d.par_const = pool.upload(cond.cst, stream);
     }
     d.rt = pool.upload(rt, stream);
-    d.tcell = pool.upload(tcell, stream);
+    d.tcode = pool.upload(tcode, stream);
     d.subj_off = pool.upload(subj_off, stream);
     d.subj_job = pool.upload(subj_job, stream);
     d.subj_unit = pool.upload(subj_unit, stream);
@@ -420,9 +457,9 @@ void ggdmc_session::build(const ggdmc_config *c, const ggdmc_model *model, const
     const int limit = device_lds_limit();
     chain_batch = ng;
     while (chain_batch > 1 &&
-           ggdmc_internal_sweep_lds(n, P, ng, d.n_cell, d.n_acc, chain_batch) > (size_t)std::min(limit, 48 * 1024))
+           ggdmc_internal_sweep_lds(n, P, ng, d.n_cond, d.n_acc, chain_batch) > (size_t)std::min(limit, 48 * 1024))
         --chain_batch;
-    sweep_lds = ggdmc_internal_sweep_lds(n, P, ng, d.n_cell, d.n_acc, chain_batch);
+    sweep_lds = ggdmc_internal_sweep_lds(n, P, ng, d.n_cond, d.n_acc, chain_batch);
     require(sweep_lds <= (size_t)limit, "design too large for LDS");
     phi_lds = ggdmc_internal_phi_lds(n, P);
     require(phi_lds <= (size_t)limit, "population state too large for LDS");
@@ -879,8 +916,8 @@ int ggdmc_loglike(const ggdmc_model *model, const ggdmc_subject *data, const dou
         require(precision == GGDMC_FP64_PLAIN || precision == GGDMC_MIXED_BATCHED,
                 "loglike precision must be FP64_PLAIN or MIXED_BATCHED");
         require(model && model->winner && model->par_src && model->par_const, "model missing");
-        require(model->n_cell >= 1 && model->n_cell <= 256 && model->n_acc >= 1 &&
-                    model->n_cell * model->n_acc <= 256 && model->npar >= 1 && model->npar <= 64,
+        require(model->n_cell >= 1 && model->n_cell <= 4096 && model->n_acc >= 1 &&
+                    model->n_acc <= 8 && model->npar >= 1 && model->npar <= 64,
                 "design out of range");
         require(data && data->cell_off && (data->rt || data->ntrial == 0), "subject data missing");
         require(data->cell_off[0] == 0 && data->cell_off[model->n_cell] == data->ntrial,
@@ -898,26 +935,28 @@ int ggdmc_loglike(const ggdmc_model *model, const ggdmc_subject *data, const dou
             require(data->cell_off[c + 1] >= data->cell_off[c], "cell_off must be non-decreasing");
         }
         /* one packed device buffer: design, trials, thetas, output */
+        const Condensed cond = condense(model);
+        require(cond.n_cond * na <= 256, "design has more than 256 condition x accumulator pairs");
         const int64_t T = data->ntrial;
-        std::vector<uint16_t> tcell((size_t)T);
+        std::vector<uint16_t> tcode((size_t)T);
         for (int c = 0; c < nc; ++c)
-            for (int64_t i = data->cell_off[c]; i < data->cell_off[c + 1]; ++i) tcell[i] = (uint16_t)c;
+            for (int64_t i = data->cell_off[c]; i < data->cell_off[c + 1]; ++i)
+                tcode[i] = (uint16_t)((cond.cond_of_cell[c] << 3) | model->winner[c]);
+        const int nq = cond.n_cond;
         const int64_t soff[2] = {0, T};
         auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-        const size_t o_src = 0, o_const = o_src + al(sizeof(int32_t) * nc * na * 6);
-        const size_t o_win = o_const + al(sizeof(double) * nc * na * 6);
-        const size_t o_rt = o_win + al(sizeof(int32_t) * nc);
+        const size_t o_src = 0, o_const = o_src + al(sizeof(int32_t) * nq * na * 6);
+        const size_t o_rt = o_const + al(sizeof(double) * nq * na * 6);
         const size_t o_cell = o_rt + al(sizeof(double) * (size_t)T);
         const size_t o_off = o_cell + al(sizeof(uint16_t) * (size_t)T);
         const size_t o_th = o_off + al(sizeof(soff));
         const size_t o_out = o_th + al(sizeof(double) * (size_t)n * P);
         const size_t total = o_out + al(sizeof(double) * (size_t)n);
         std::vector<char> h(total);
-        std::memcpy(&h[o_src], model->par_src, sizeof(int32_t) * nc * na * 6);
-        std::memcpy(&h[o_const], model->par_const, sizeof(double) * nc * na * 6);
-        std::memcpy(&h[o_win], model->winner, sizeof(int32_t) * nc);
+        std::memcpy(&h[o_src], cond.src.data(), sizeof(int32_t) * nq * na * 6);
+        std::memcpy(&h[o_const], cond.cst.data(), sizeof(double) * nq * na * 6);
         if (T) std::memcpy(&h[o_rt], data->rt, sizeof(double) * (size_t)T);
-        if (T) std::memcpy(&h[o_cell], tcell.data(), sizeof(uint16_t) * (size_t)T);
+        if (T) std::memcpy(&h[o_cell], tcode.data(), sizeof(uint16_t) * (size_t)T);
         std::memcpy(&h[o_off], soff, sizeof(soff));
         std::memcpy(&h[o_th], theta, sizeof(double) * (size_t)n * P);
         HIPCHECK(hipMalloc(&dbuf, total));
@@ -926,16 +965,15 @@ int ggdmc_loglike(const ggdmc_model *model, const ggdmc_subject *data, const dou
         Dev d{};
         d.P = P;
         d.n_acc = na;
-        d.n_cell = nc;
+        d.n_cond = nq;
         d.nchain = 1;
         d.par_src = reinterpret_cast<const int32_t *>(b + o_src);
         d.par_const = reinterpret_cast<const double *>(b + o_const);
-        d.winner = reinterpret_cast<const int32_t *>(b + o_win);
         d.rt = reinterpret_cast<const double *>(b + o_rt);
-        d.tcell = reinterpret_cast<const uint16_t *>(b + o_cell);
+        d.tcode = reinterpret_cast<const uint16_t *>(b + o_cell);
         d.subj_off = reinterpret_cast<const int64_t *>(b + o_off);
         d.S = 1;
-        const size_t lds = ggdmc_internal_sweep_lds(1, P, 1, nc, na, 1);
+        const size_t lds = ggdmc_internal_sweep_lds(1, P, 1, nq, na, 1);
         HIPCHECK(ggdmc_internal_launch_loglike(&d, reinterpret_cast<const double *>(b + o_th), n, 0,
                                                precision, lds, nullptr,
                                                reinterpret_cast<double *>(b + o_out)));

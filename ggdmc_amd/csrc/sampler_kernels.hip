@@ -89,20 +89,19 @@ __device__ double block_sum(double v, double *red) {
 }
 
 /* Evaluates trials [t_lo, t_hi) for chains [0, kb) of a batch whose AccK
- * table is acck[kb][n_cell][n_acc]; adds each wave's per-chain sums to
+ * table is acck[kb][n_cond][n_acc]; adds each wave's per-chain sums to
  * part[wave][cb + c]. Units (chain, block of 64*TPL trials) are split into
  * contiguous ranges per wave, so a wave reduces once per chain it touches. */
 template <int MODE, int TPL>
 __device__ void eval_units(const Dev &d, int64_t t_lo, int64_t t_hi, int kb, int cb,
-                           const int *active, const double *acck, const int32_t *win,
-                           double *part, int ng_stride) {
+                           const int *active, const double *acck, double *part, int ng_stride) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nw = blockDim.x >> 6;
     const int64_t span = 64 * TPL;
     const int nblk = (int)((t_hi - t_lo + span - 1) / span);
     const int U = kb * nblk;
     const int u0 = (int)(((int64_t)wave * U) / nw), u1 = (int)(((int64_t)(wave + 1) * U) / nw);
-    const int per_chain = d.n_cell * d.n_acc;
+    const int per_chain = d.n_cond * d.n_acc;
     int cur = -1;
     double acc = 0.0;
     for (int u = u0; u < u1; ++u) {
@@ -122,8 +121,8 @@ __device__ void eval_units(const Dev &d, int64_t t_lo, int64_t t_hi, int kb, int
         for (int t = 0; t < TPL; ++t) {
             const int64_t i = base + 64 * t;
             if (i < t_hi) {
-                const int cell = d.tcell[i];
-                acc += trial_term<MODE>(kc + cell * d.n_acc, d.n_acc, win[cell], d.rt[i]);
+                const int code = d.tcode[i];
+                acc += trial_term<MODE>(kc + (code >> 3) * d.n_acc, d.n_acc, code & 7, d.rt[i]);
             }
         }
     }
@@ -184,18 +183,17 @@ __device__ __forceinline__ bool mh_accept(double tpost, double cur, double U) {
 struct SweepLds {
     double *th, *lp, *ll, *prop, *tlp, *U, *part, *tile, *acck, *red;
     int *active, *sub, *flag;
-    int32_t *win;
 };
 
-__host__ __device__ inline size_t sweep_lds_bytes(int n, int P, int ng, int n_cell, int n_acc,
+__host__ __device__ inline size_t sweep_lds_bytes(int n, int P, int ng, int n_cond, int n_acc,
                                                   int chain_batch) {
     size_t dbl = (size_t)n * P + 2 * n + (size_t)ng * P + 3 * ng + (size_t)kWaves * ng +
-                 (size_t)chain_batch * n_cell * n_acc * kAccKDoubles + kWaves;
-    size_t ints = (size_t)ng + n + 4 + n_cell;
+                 (size_t)chain_batch * n_cond * n_acc * kAccKDoubles + kWaves;
+    size_t ints = (size_t)ng + n + 4;
     return dbl * sizeof(double) + ints * sizeof(int);
 }
 
-__device__ SweepLds carve(char *base, int n, int P, int ng, int n_cell, int n_acc, int chain_batch) {
+__device__ SweepLds carve(char *base, int n, int P, int ng, int n_cond, int n_acc, int chain_batch) {
     SweepLds L;
     double *p = reinterpret_cast<double *>(base);
     L.th = p; p += (size_t)n * P;
@@ -206,19 +204,18 @@ __device__ SweepLds carve(char *base, int n, int P, int ng, int n_cell, int n_ac
     L.U = p; p += ng;
     L.tile = p; p += ng;
     L.part = p; p += (size_t)kWaves * ng;
-    L.acck = p; p += (size_t)chain_batch * n_cell * n_acc * kAccKDoubles;
+    L.acck = p; p += (size_t)chain_batch * n_cond * n_acc * kAccKDoubles;
     L.red = p; p += kWaves;
     int *q = reinterpret_cast<int *>(p);
     L.active = q; q += ng;
     L.sub = q; q += n;
-    L.flag = q; q += 4;
-    L.win = q;
+    L.flag = q;
     return L;
 }
 
 /* Fills the AccK tables of chains [cb, cb+kb) of the proposal batch. */
 __device__ void build_acck(const Dev &d, const double *prop, int P, int cb, int kb, double *acck) {
-    const int per_chain = d.n_cell * d.n_acc;
+    const int per_chain = d.n_cond * d.n_acc;
     AccK *K = reinterpret_cast<AccK *>(acck);
     for (int e = threadIdx.x; e < kb * per_chain; e += blockDim.x) {
         const int c = e / per_chain, r = e - c * per_chain;
@@ -238,7 +235,7 @@ __device__ void tile_loglike(const Dev &d, const SweepLds &L, int ng, int chain_
         __syncthreads();
         build_acck(d, L.prop, P, cb, kb, L.acck);
         __syncthreads();
-        eval_units<MODE, TPL>(d, t_lo, t_hi, kb, cb, L.active, L.acck, L.win, L.part, ng);
+        eval_units<MODE, TPL>(d, t_lo, t_hi, kb, cb, L.active, L.acck, L.part, ng);
     }
     __syncthreads();
     for (int c = threadIdx.x; c < ng; c += blockDim.x) {
@@ -323,7 +320,7 @@ __global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs 
     const JobDev job = d.jobs[d.subj_job[s]];
     const int n = d.nchain, P = d.P, G = d.G;
     const int ng_max = (n + G - 1) / G + 1;
-    const SweepLds L = carve(lds_raw, n, P, ng_max, d.n_cell, d.n_acc, a.chain_batch);
+    const SweepLds L = carve(lds_raw, n, P, ng_max, d.n_cond, d.n_acc, a.chain_batch);
     const uint32_t iter = *d.iter_base + a.iter_off;
     const bool single = job.kind != GGDMC_KIND_HIER;
     const bool blocked = d.blocked_sub != 0;
@@ -341,7 +338,6 @@ __global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs 
         L.lp[c] = d.lp[(size_t)s * n + c];
         L.ll[c] = d.ll[(size_t)s * n + c];
     }
-    for (int c = threadIdx.x; c < d.n_cell; c += blockDim.x) L.win[c] = d.winner[c];
     __syncthreads();
 
     if (migrate) {
@@ -753,8 +749,7 @@ __device__ void fused_phi_tail(const Dev &d, int s, const JobDev &job, uint32_t 
 __global__ __launch_bounds__(kBlock) void plain_subject_kernel(Dev d, uint32_t iter, int sweep, int j) {
     __shared__ double prop[64];
     __shared__ double red[kWaves];
-    __shared__ AccK acck[64 * 4];
-    __shared__ int32_t win[256];
+    __shared__ AccK acck[256];
     __shared__ int sub[kMaxChains];
     __shared__ int info[4];
     __shared__ double misc[2];
@@ -807,17 +802,16 @@ __global__ __launch_bounds__(kBlock) void plain_subject_kernel(Dev d, uint32_t i
             info[1] = j;
         }
     }
-    for (int c = threadIdx.x; c < d.n_cell; c += blockDim.x) win[c] = d.winner[c];
     __syncthreads();
     if (!info[0]) return;
-    for (int e = threadIdx.x; e < d.n_cell * d.n_acc; e += blockDim.x)
+    for (int e = threadIdx.x; e < d.n_cond * d.n_acc; e += blockDim.x)
         make_acck<DevOps>(prop, d.par_src + e * 6, d.par_const + e * 6, acck[e]);
     __syncthreads();
     const int64_t t_lo = d.subj_off[s], t_hi = d.subj_off[s + 1];
     double acc = 0.0;
     for (int64_t i = t_lo + threadIdx.x; i < t_hi; i += blockDim.x) {
-        const int cell = d.tcell[i];
-        acc += trial_term<MODE_NAIVE>(acck + cell * d.n_acc, d.n_acc, win[cell], d.rt[i]);
+        const int code = d.tcode[i];
+        acc += trial_term<MODE_NAIVE>(acck + (code >> 3) * d.n_acc, d.n_acc, code & 7, d.rt[i]);
     }
     const double tll = block_sum(acc, red);
     if (threadIdx.x == 0) {
@@ -1020,10 +1014,9 @@ __global__ __launch_bounds__(kBlock) void loglike_kernel(Dev d, const double *th
                                                         double *out) {
     extern __shared__ __attribute__((aligned(16))) char lds_raw[];
     const int P = d.P;
-    const SweepLds L = carve(lds_raw, 1, P, 1, d.n_cell, d.n_acc, 1);
+    const SweepLds L = carve(lds_raw, 1, P, 1, d.n_cond, d.n_acc, 1);
     const int q = blockIdx.x;
     for (int p = threadIdx.x; p < P; p += blockDim.x) L.prop[p] = thetas[(size_t)q * P + p];
-    for (int c = threadIdx.x; c < d.n_cell; c += blockDim.x) L.win[c] = d.winner[c];
     if (threadIdx.x == 0) L.active[0] = 1;
     __syncthreads();
     tile_loglike<MODE, TPL>(d, L, 1, 1, d.subj_off[s], d.subj_off[s + 1]);
@@ -1037,8 +1030,8 @@ using namespace ggdmc;
 
 extern "C" {
 
-size_t ggdmc_internal_sweep_lds(int n, int P, int ng, int n_cell, int n_acc, int chain_batch) {
-    return sweep_lds_bytes(n, P, ng, n_cell, n_acc, chain_batch);
+size_t ggdmc_internal_sweep_lds(int n, int P, int ng, int n_cond, int n_acc, int chain_batch) {
+    return sweep_lds_bytes(n, P, ng, n_cond, n_acc, chain_batch);
 }
 size_t ggdmc_internal_phi_lds(int n, int P) { return phi_lds_bytes(n, P); }
 
