@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--profile-iters", type=int, default=20)
+    ap.add_argument("--decomp", default="",
+                    help="fixed work items 'k,t' (chains per item or -1, trials per item or 0) "
+                         "instead of autotuning")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device-to-device copy (counter calibration)")
     return ap.parse_args()
@@ -126,12 +129,19 @@ def main():
     total = a.warmup + a.steps + a.profile_iters
     thin = max(1, total // 1000)
     nmc = total // thin + 2
-    cfg = w.config(nmc=nmc, thin=thin)
+    if a.decomp:
+        k, t = (int(v) for v in a.decomp.split(","))
+        cfg = w.config(nmc=nmc, thin=thin, chains_per_item=k, trials_per_tile=t)
+    else:
+        cfg = w.config(nmc=nmc, thin=thin)
     sess = g.Session(cfg, w.model, w.jobs)
-    tpt, ntiles = sess.tile()
-    print("[bench] workload %s: %s; %d subjects, %d trials; tile %d trials, %d tiles; "
-          "setup %.1fs" % (w.name, w.description, w.n_subjects, w.n_trials, tpt, ntiles,
-                           time.time() - t_build), file=sys.stderr, flush=True)
+    kci, tpt, nitems = sess.decomposition()
+    print("[bench] decomp %d,%d" % (kci, tpt), file=sys.stderr, flush=True)
+    print("[bench] workload %s: %s; %d subjects, %d trials; items: %s chains x %s trials, "
+          "%d per launch; setup %.1fs"
+          % (w.name, w.description, w.n_subjects, w.n_trials,
+             "all groups" if kci < 0 else kci, tpt or "all", nitems, time.time() - t_build),
+          file=sys.stderr, flush=True)
 
     sess.advance(a.warmup)
     e0 = sess.trial_evals()
@@ -158,8 +168,9 @@ def main():
     res = sess.download()
     sess.close()
 
-    # per-proposal |dLL|: stored subject states against the FP64 path (every
-    # subject for up to 64 subjects, an even subsample beyond)
+    # per-proposal |dLL|: the last stored state of every chain against the
+    # FP64 path (every subject for up to 64 subjects, an even subsample
+    # beyond); a non-finite ratio counts as a failure
     worst = 0.0
     n_cmp = 0
     pairs = []
@@ -167,10 +178,11 @@ def main():
         datas = [job.subject] if job.kind == g.KIND_SUBJECT else job.subjects
         pairs.extend(zip(datas, r["subjects"]))
     step = max(1, len(pairs) // 64)
+    k_last = total // thin          # iterations 1..total ran; column i/thin holds iteration i
     for d, post in pairs[::step]:
-        k = post.nmc - 1
-        ref = g.loglike(w.model, d, post.theta[:, :, k].T, g.FP64_PLAIN)
-        worst = max(worst, float(np.max(np.abs(post.log_likelihoods[:, k] / ref - 1))))
+        ref = g.loglike(w.model, d, post.theta[:, :, k_last].T, g.FP64_PLAIN)
+        r = np.abs(post.log_likelihoods[:, k_last] / ref - 1)
+        worst = max(worst, float(np.max(r)) if np.all(np.isfinite(r)) else float("inf"))
         n_cmp += len(ref)
 
     plain = None
@@ -200,7 +212,7 @@ def main():
         "dtype": "fp32 transcendentals + fp64 sums/state", "data": "synthetic",
         "config": {"workload": "%s: %s" % (w.name, w.description), "subjects": w.n_subjects,
                    "trials": w.n_trials, "chains": w.nchain, "sweep_groups": 2,
-                   "trials_per_tile": tpt, "tiles": ntiles},
+                   "chains_per_item": kci, "trials_per_item": tpt, "items_per_launch": nitems},
         "trial_evals_per_s": round(evals_per_s, 1),
         "plain_fp64_iter_per_s": round(plain, 3) if plain else None,
         "speedup_vs_plain_fp64": round(iters_per_s / plain, 2) if plain else None,

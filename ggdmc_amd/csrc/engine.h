@@ -10,8 +10,16 @@
  *   theta   double [S][n][P]    current subject states (chain-major, P contiguous)
  *   lp, ll  double [S][n]
  *   phi     double [F][n][2P]   population states of hierarchical / hyper runs
- *   partial double [tiles][n]   per-tile partial log-likelihoods of the current group
+ *   stats   double [F][n][P][4] per-chain sufficient statistics of the subject states
+ *   csum    double [chunks][n]  chunk sums of trial-split work items
  *   out_*   [S][nmc][n][P] etc. stored samples, R's column-major order per subject
+ *
+ * Log-likelihood sums are canonical: chunk j of a subject is its trials
+ * [j * kChunk, (j + 1) * kChunk); a 64-lane wave sums a chunk with lane l
+ * holding term(j*kChunk + l) + term(j*kChunk + 64 + l) and a fixed xor
+ * butterfly; a chain's log-likelihood is the sequential sum of its chunk sums
+ * in chunk order. How chains and trials are split over workgroups therefore
+ * never changes a result bit.
  */
 #ifndef GGDMC_ENGINE_H
 #define GGDMC_ENGINE_H
@@ -22,7 +30,8 @@ namespace ggdmc {
 
 constexpr int kBlock = 256;        /* threads per workgroup of the sweep kernels */
 constexpr int kWaves = kBlock / 64;
-constexpr int kMaxChains = 256;    /* chains per run handled by one workgroup */
+constexpr int kMaxChains = 256;    /* chains per run */
+constexpr int kChunk = 128;        /* trials per canonical chunk sum */
 
 struct JobDev {
     int32_t kind;           /* GGDMC_KIND_* */
@@ -38,7 +47,8 @@ struct Dev {
     int32_t P, n_acc, n_cond, nchain, G, nmc, thin;
     int32_t blocked_sub, blocked_pop;
     double rp, sub_mig, pop_mig;
-    int32_t S, n_tiles, n_phi, n_jobs;
+    int32_t S, n_phi, n_jobs;
+    int32_t tpt;                /* trials per tile of trial-split items (multiple of kChunk) */
     uint32_t philox_offset;     /* Philox iteration = local iteration + this */
     uint32_t last_iter;         /* last local iteration of the run (nmc*thin - 1) */
 
@@ -48,12 +58,9 @@ struct Dev {
     const double *rt;
     const uint16_t *tcode;
     const int64_t *subj_off;    /* [S+1] trial offsets */
+    const int64_t *chunk_off;   /* [S+1] chunk offsets (rows of csum) */
     const int32_t *subj_job;    /* [S] */
     const int32_t *subj_unit;   /* [S] Philox unit (index within its run) */
-    const int32_t *subj_tile0;  /* [S] first tile */
-    const int32_t *subj_ntile;  /* [S] */
-    const int32_t *tile_subj;   /* [tiles] */
-    const int64_t *tile_lo, *tile_hi;
 
     const JobDev *jobs;
     const int32_t *phi_job;     /* [F] job of each population state */
@@ -64,25 +71,39 @@ struct Dev {
     double *theta, *lp, *ll;
     double *phi, *phi_lp, *phi_ll;
     double *stats;              /* [F][n][P][4]: mean, M2, min, max over subjects */
+    double *csum;
 
-    double *partial;
-    uint32_t *counter;          /* [S] tile arrivals, [F] stats arrivals, [jobs] subject arrivals */
-    unsigned long long *evals;  /* trial-likelihood evaluations */
+    uint32_t *subj_ctr;         /* [S] items of a subject finished (final launch) */
+    uint32_t *stats_ctr;        /* [F] statistics blocks finished */
+    uint32_t *job_ctr;          /* [jobs] subjects of a run finished */
+    uint32_t *tile_ctr;         /* [items] tiles of a (subject, chain range) finished */
+    unsigned long long *evals;  /* [S] trial-likelihood evaluations per subject */
     const uint32_t *iter_base;  /* device iteration base (graph replays) */
 
     double *out_theta, *out_lp, *out_ll;
     double *phi_out_theta, *phi_out_lp, *phi_out_ll;
 };
 
+/* A work item of a subject launch: subject x, chains [c0, c1) of the chain
+ * group (c1 = kAllChains: to the group's end), trial tile t of ntile, and w =
+ * the number of items of subject x in the launch. Packed as int4:
+ * {x, c0 << 16 | c1, t << 16 | ntile, w}. The tiles of one (x, c0) are
+ * consecutive, in tile order. */
+constexpr int kAllChains = 0xFFFF;
+
 /* Per-launch arguments of the subject sweep kernel. */
 struct SweepArgs {
-    uint32_t iter_off;   /* iteration = *iter_base + iter_off */
-    int32_t sweep;       /* blocked parameter index or 0 */
+    uint32_t iter_off;    /* iteration = *iter_base + iter_off */
+    int32_t sweep;        /* blocked parameter index or 0 */
     int32_t g_begin, g_end;
     int32_t final_launch; /* last subject launch of the iteration: store if due */
-    int32_t chain_batch;  /* chains whose constants fit in LDS at once */
+    int32_t chain_batch;  /* chains whose accumulator constants fit in LDS at once */
     int32_t fuse_phi;     /* final launch: the last subject of a hierarchical run
                              performs the next iteration's population step */
+    int32_t kmax;         /* most chains one item proposes */
+    int32_t cmax;         /* chunk sums per chain held in LDS at once */
+    int32_t n_items;
+    const int4 *items;    /* grid = one workgroup per item */
 };
 
 } // namespace ggdmc

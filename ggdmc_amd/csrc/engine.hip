@@ -3,11 +3,12 @@
  *
  * A session uploads a batch of runs once (trials, designs, priors, start
  * states), keeps every state in HBM, and advances all runs together:
- *   batched path : per iteration one population launch (hierarchical /
- *                  hyper runs) and one subject launch per sweep (all chain
- *                  groups in-kernel when every subject is one tile) or per
- *                  (sweep, group); blocks of iterations are captured in a
- *                  hipGraph and replayed.
+ *   batched path : per iteration one subject launch per (sweep, chain group)
+ *                  over work items (subject, chain range, trial tile), or one
+ *                  per sweep with one item per subject over every group; the
+ *                  population step runs in its own launch or at the end of
+ *                  the last subject launch; blocks of iterations are captured
+ *                  in a hipGraph and replayed.
  *   plain path   : the reference's sequential order, one launch per chain
  *                  step, no graph (the FP64 comparator).
  * The reference's own driver loop is de_class::run_chains / run_hchains
@@ -30,10 +31,10 @@
 using namespace ggdmc;
 
 extern "C" {
-size_t ggdmc_internal_sweep_lds(int n, int P, int ng, int n_cell, int n_acc, int chain_batch);
-size_t ggdmc_internal_phi_lds(int n, int P);
-hipError_t ggdmc_internal_launch_sweep(const Dev *d, const SweepArgs *a, int mode, int tpl,
-                                       size_t lds, hipStream_t stream);
+size_t ggdmc_internal_sweep_lds(int n, int P, int kmax, int per_chain, int chain_batch, int cmax);
+size_t ggdmc_internal_phi_lds(int n, int P, int G);
+hipError_t ggdmc_internal_launch_sweep(const Dev *d, const SweepArgs *a, int mode, size_t lds,
+                                       hipStream_t stream);
 hipError_t ggdmc_internal_launch_phi(const Dev *d, uint32_t iter_off, size_t lds, hipStream_t stream);
 hipError_t ggdmc_internal_launch_plain_subject(const Dev *d, uint32_t iter, int sweep, int j,
                                                hipStream_t stream);
@@ -44,7 +45,8 @@ hipError_t ggdmc_internal_launch_store(const Dev *d, uint32_t iter, hipStream_t 
 hipError_t ggdmc_internal_launch_hyper_stats(const Dev *d, const double *rows, int S, int f,
                                              hipStream_t stream);
 hipError_t ggdmc_internal_launch_loglike(const Dev *d, const double *thetas, int nq, int s,
-                                         int mode, size_t lds, hipStream_t stream, double *out);
+                                         int mode, int cmax, size_t lds, hipStream_t stream,
+                                         double *out);
 }
 
 namespace {
@@ -110,12 +112,16 @@ struct ggdmc_session {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     Dev d{};
     int mode = GGDMC_MIXED_BATCHED;
-    int tpl = 2;
-    int tpt = 0;                 /* trials per tile; 0 = whole subject */
-    int chain_batch = 0;
+    /* work decomposition of the subject launches */
+    bool fused = false;          /* one item per subject over every chain group */
+    int kchain = 0;              /* chains per item */
+    int tpt = 0;                 /* trials per item; 0 = whole subject */
+    int kmax = 1, cmax = 1, chain_batch = 1;
+    std::vector<const int4 *> lists;   /* item list per launch (per group, or one) */
+    std::vector<int> list_n;
     size_t sweep_lds = 0, phi_lds = 0;
     int nsweep_sub = 1;
-    bool fused_groups = true;
+    int64_t total_chunks = 0;
     bool fuse_phi = false;       /* population step fused into the final subject launch */
     uint32_t phi_ready = 0;      /* iteration whose population step has been done */
     uint32_t next_iter = 1;      /* next local iteration to run (1-based) */
@@ -143,7 +149,8 @@ struct ggdmc_session {
     }
 
     void build(const ggdmc_config *c, const ggdmc_model *model, const ggdmc_job *js, int nj);
-    void set_tiles(int tpt_new);
+    void set_decomp(bool fused_new, int k_new, int tpt_new);
+    void default_decomp();
     void enqueue_iteration(uint32_t off, std::vector<hipEvent_t> *ev = nullptr);
     void ensure_phi_ready();
     void run_batched(uint32_t n_iter);
@@ -396,10 +403,22 @@ void ggdmc_session::build(const ggdmc_config *c, const ggdmc_model *model, const
     d.phi_lp = pool.upload(plp0, stream);
     d.phi_ll = pool.upload(pll0, stream);
     d.stats = pool.alloc<double>((size_t)std::max(F, 1) * n * P * 4);
-    d.counter = pool.alloc<uint32_t>((size_t)S + F + nj + 1);
-    HIPCHECK(hipMemsetAsync(d.counter, 0, sizeof(uint32_t) * (S + F + nj + 1), stream));
-    d.evals = pool.alloc<unsigned long long>(1);
-    HIPCHECK(hipMemsetAsync(d.evals, 0, sizeof(unsigned long long), stream));
+    {
+        uint32_t *ctr = pool.alloc<uint32_t>((size_t)S + F + nj);
+        HIPCHECK(hipMemsetAsync(ctr, 0, sizeof(uint32_t) * ((size_t)S + F + nj), stream));
+        d.subj_ctr = ctr;
+        d.stats_ctr = ctr + S;
+        d.job_ctr = ctr + S + F;
+    }
+    d.evals = pool.alloc<unsigned long long>((size_t)std::max(S, 1));
+    HIPCHECK(hipMemsetAsync(d.evals, 0, sizeof(unsigned long long) * std::max(S, 1), stream));
+    {
+        std::vector<int64_t> chunk_off{0};
+        for (int x = 0; x < S; ++x) chunk_off.push_back(chunk_off.back() + (subj_ntr[x] + kChunk - 1) / kChunk);
+        total_chunks = chunk_off.back();
+        d.chunk_off = pool.upload(chunk_off, stream);
+        d.csum = pool.alloc<double>((size_t)std::max<int64_t>(total_chunks, 1) * n);
+    }
     iter_base = pool.alloc<uint32_t>(1);
     d.iter_base = iter_base;
     hyper_rows = pool.upload(hyper_rows_h, stream);
@@ -452,54 +471,87 @@ void ggdmc_session::build(const ggdmc_config *c, const ggdmc_model *model, const
         fuse_phi = mode != GGDMC_FP64_PLAIN && F > 0 && !any_hyper && work <= 65536;
     }
 
-    /* LDS budget: chains whose AccK tables are resident at once */
-    const int ng = (n + G - 1) / G + 1;
-    const int limit = device_lds_limit();
-    chain_batch = ng;
-    while (chain_batch > 1 &&
-           ggdmc_internal_sweep_lds(n, P, ng, d.n_cond, d.n_acc, chain_batch) > (size_t)std::min(limit, 48 * 1024))
-        --chain_batch;
-    sweep_lds = ggdmc_internal_sweep_lds(n, P, ng, d.n_cond, d.n_acc, chain_batch);
-    require(sweep_lds <= (size_t)limit, "design too large for LDS");
-    phi_lds = ggdmc_internal_phi_lds(n, P);
-    require(phi_lds <= (size_t)limit, "population state too large for LDS");
-    if (fuse_phi) sweep_lds = std::max(sweep_lds, phi_lds);
-
-    tpl = 2;
-    set_tiles(cfg.trials_per_tile > 0 ? cfg.trials_per_tile : 0);
+    phi_lds = ggdmc_internal_phi_lds(n, P, G);
+    require(F == 0 || phi_lds <= (size_t)device_lds_limit(), "population state too large for LDS");
+    default_decomp();
     HIPCHECK(hipStreamSynchronize(stream));
 }
 
-/* Splits every subject's trials into tiles of tpt_new trials (rounded up to a
- * multiple of 64; 0 = one tile per subject). */
-void ggdmc_session::set_tiles(int tpt_new) {
-    if (tpt_new > 0) tpt_new = ((tpt_new + 63) / 64) * 64;
-    tpt = tpt_new;
-    std::vector<int32_t> tile_subj, tile0, ntile;
-    std::vector<int64_t> lo, hi;
-    int64_t off = 0;
-    for (int s = 0; s < d.S; ++s) {
-        const int64_t ntr = subj_ntr[s];
-        const int64_t nt = (tpt == 0 || ntr <= tpt) ? 1 : (ntr + tpt - 1) / tpt;
-        tile0.push_back((int32_t)tile_subj.size());
-        ntile.push_back((int32_t)nt);
-        for (int64_t t = 0; t < nt; ++t) {
-            tile_subj.push_back(s);
-            const int64_t a = off + (nt == 1 ? 0 : t * tpt);
-            const int64_t b = nt == 1 ? off + ntr : std::min(off + ntr, a + tpt);
-            lo.push_back(a);
-            hi.push_back(b);
-        }
-        off += ntr;
+/* The decomposition the configuration asks for, or (auto) enough items to
+ * fill the chip: chains per item so that subjects x chain ranges >= 1024. */
+void ggdmc_session::default_decomp() {
+    const int ngmax = (d.nchain + d.G - 1) / d.G;
+    if (cfg.chains_per_item < 0) {
+        require(cfg.trials_per_tile == 0, "one item per subject needs trials_per_tile = 0");
+        set_decomp(true, 0, 0);
+    } else if (cfg.chains_per_item > 0 || cfg.trials_per_tile > 0) {
+        set_decomp(false, cfg.chains_per_item, cfg.trials_per_tile);
+    } else {
+        int k = ngmax;
+        while (k > 1 && (int64_t)d.S * ((ngmax + k - 1) / k) < 1024) k = (k + 1) / 2;
+        set_decomp(false, k, 0);
     }
-    fused_groups = std::all_of(ntile.begin(), ntile.end(), [](int32_t v) { return v == 1; });
-    d.n_tiles = (int)tile_subj.size();
-    d.tile_subj = pool.upload(tile_subj, stream);
-    d.subj_tile0 = pool.upload(tile0, stream);
-    d.subj_ntile = pool.upload(ntile, stream);
-    d.tile_lo = pool.upload(lo, stream);
-    d.tile_hi = pool.upload(hi, stream);
-    d.partial = pool.alloc<double>((size_t)std::max(d.n_tiles, 1) * d.nchain);
+}
+
+/* Builds the item lists of the subject launches: per chain group, every
+ * subject's chain ranges of k_new chains x trial tiles of tpt_new trials
+ * (rounded up to a multiple of kChunk; 0 = whole subject); or, fused, one
+ * item per subject that sweeps every group in one launch. Sizes the LDS. */
+void ggdmc_session::set_decomp(bool fused_new, int k_new, int tpt_new) {
+    if (tpt_new > 0) tpt_new = ((tpt_new + kChunk - 1) / kChunk) * kChunk;
+    require(!fused_new || tpt_new == 0, "one item per subject needs whole subjects");
+    const int n = d.nchain, G = d.G;
+    const int ngmax = (n + G - 1) / G;
+    fused = fused_new;
+    tpt = tpt_new;
+    d.tpt = tpt;
+    kchain = fused ? ngmax : std::min(k_new > 0 ? k_new : ngmax, ngmax);
+    lists.clear();
+    list_n.clear();
+    size_t max_items = 1;
+    int64_t max_chunks_item = 1;
+    const int nl = fused ? 1 : G;
+    for (int g = 0; g < nl; ++g) {
+        const int ng = fused ? ngmax : (((g + 1) * n) / G - (g * n) / G);
+        const int nch = fused ? 1 : (ng + kchain - 1) / kchain;
+        std::vector<int4> items;
+        for (int x = 0; x < d.S; ++x) {
+            const int64_t ntr = subj_ntr[x];
+            const int64_t nt = (tpt == 0 || ntr <= tpt) ? 1 : (ntr + tpt - 1) / tpt;
+            require(nt <= 65535, "trials_per_tile too small for a subject's trial count");
+            const int64_t w = (int64_t)nch * nt;
+            require(w < (int64_t)1 << 30, "too many work items per subject");
+            for (int ch = 0; ch < nch; ++ch) {
+                const int c0 = ch * kchain;
+                const int c1 = (fused || ch == nch - 1) ? kAllChains : c0 + kchain;
+                for (int64_t t = 0; t < nt; ++t)
+                    items.push_back(make_int4(x, (c0 << 16) | c1, ((int)t << 16) | (int)nt, (int)w));
+            }
+            max_chunks_item = std::max<int64_t>(max_chunks_item,
+                                                nt == 1 ? (ntr + kChunk - 1) / kChunk : tpt / kChunk);
+        }
+        require(items.size() < ((size_t)1 << 31), "too many work items");
+        list_n.push_back((int)items.size());
+        lists.push_back(pool.upload(items, stream));
+        max_items = std::max(max_items, items.size());
+    }
+    d.tile_ctr = pool.alloc<uint32_t>(max_items);
+    HIPCHECK(hipMemsetAsync(d.tile_ctr, 0, sizeof(uint32_t) * max_items, stream));
+    /* LDS: accumulator constants of chain_batch chains and cmax chunk sums
+     * per chain at once, within 48 KB when possible */
+    kmax = kchain;
+    cmax = (int)std::min<int64_t>(max_chunks_item, std::max(1, 4096 / kmax));
+    chain_batch = kmax;
+    const int per_chain = d.n_cond * d.n_acc;
+    auto bytes = [&]() {
+        return ggdmc_internal_sweep_lds(n, d.P, kmax, per_chain, chain_batch, cmax);
+    };
+    const size_t soft = 48 * 1024, limit = (size_t)device_lds_limit();
+    while (chain_batch > 1 && bytes() > soft) --chain_batch;
+    while (cmax > 1 && bytes() > soft) cmax = (cmax + 1) / 2;
+    sweep_lds = bytes();
+    require(sweep_lds <= limit, "design too large for LDS (chains x parameters)");
+    if (fuse_phi) sweep_lds = std::max(sweep_lds, phi_lds);
     drop_graph();
 }
 
@@ -511,12 +563,12 @@ void ggdmc_session::enqueue_iteration(uint32_t off, std::vector<hipEvent_t> *ev)
     if (d.n_phi > 0 && !fuse_phi) HIPCHECK(ggdmc_internal_launch_phi(&d, off, phi_lds, stream));
     if (d.S == 0) return;
     const int lk = mode == GGDMC_FP64_BATCHED ? GGDMC_FP64_BATCHED : GGDMC_MIXED_BATCHED;
-    const int ngl = fused_groups ? 1 : d.G;
+    const int ngl = fused ? 1 : d.G;
     for (int sw = 0; sw < nsweep_sub; ++sw) {
         for (int g = 0; g < ngl; ++g) {
             const int fin = sw == nsweep_sub - 1 && g == ngl - 1;
-            SweepArgs a{off, sw, fused_groups ? 0 : g, fused_groups ? d.G : g + 1, fin, chain_batch,
-                        (int)(fin && fuse_phi)};
+            SweepArgs a{off, sw, fused ? 0 : g, fused ? d.G : g + 1, fin, chain_batch,
+                        (int)(fin && fuse_phi), kmax, cmax, list_n[g], lists[g]};
             hipEvent_t e0 = nullptr, e1 = nullptr;
             if (ev) {
                 HIPCHECK(hipEventCreate(&e0));
@@ -525,7 +577,7 @@ void ggdmc_session::enqueue_iteration(uint32_t off, std::vector<hipEvent_t> *ev)
                 ev->push_back(e1);
                 HIPCHECK(hipEventRecord(e0, stream));
             }
-            HIPCHECK(ggdmc_internal_launch_sweep(&d, &a, lk, tpl, sweep_lds, stream));
+            HIPCHECK(ggdmc_internal_launch_sweep(&d, &a, lk, sweep_lds, stream));
             if (ev) HIPCHECK(hipEventRecord(e1, stream));
         }
     }
@@ -602,74 +654,81 @@ void ggdmc_session::run_plain(uint32_t n_iter) {
     }
 }
 
-/* Picks (trials per tile, trials per lane) by timing a few real iterations
- * of each candidate from a snapshot of the state, which is then restored. */
+/* Picks the work decomposition by timing graph replays of a few real
+ * iterations of each candidate from a snapshot of the state, which is then
+ * restored (results do not depend on the decomposition, only the time does). */
 void ggdmc_session::autotune() {
     if (d.S == 0) return;
-    std::vector<int> cands{0};
-    for (int t = 64; t < max_ntr; t *= 2) cands.push_back(t);
+    struct Cand {
+        bool fused;
+        int k, tpt;
+    };
+    const int ngmax = (d.nchain + d.G - 1) / d.G;
+    std::vector<Cand> cands{{true, 0, 0}};
+    std::vector<int> ks;
+    for (int k : {1, 2, 4, 8, ngmax})
+        if (k <= ngmax && std::find(ks.begin(), ks.end(), k) == ks.end()) ks.push_back(k);
+    for (int k : ks)
+        for (int t : {0, 256, 1024})
+            if (t == 0 || t < max_ntr) {
+                const int64_t nt = t == 0 ? 1 : (max_ntr + t - 1) / t;
+                if ((int64_t)d.S * ((ngmax + k - 1) / k) * nt <= ((int64_t)1 << 22)) cands.push_back({false, k, t});
+            }
     const size_t nth = (size_t)d.S * d.nchain * d.P, nch = (size_t)d.S * d.nchain;
     const size_t nph = (size_t)d.n_phi * d.nchain * 2 * d.P, npc = (size_t)d.n_phi * d.nchain;
+    const size_t nev = (size_t)std::max(d.S, 1);
     double *snap = pool.alloc<double>(nth + 2 * nch + nph + 2 * npc);
-    unsigned long long ev_snap = 0;
+    unsigned long long *ev_snap = pool.alloc<unsigned long long>(nev);
+    auto copy = [&](void *dst, const void *src, size_t bytes) {
+        if (bytes) HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream));
+    };
     auto save = [&]() {
-        HIPCHECK(hipMemcpyAsync(snap, d.theta, nth * 8, hipMemcpyDeviceToDevice, stream));
-        HIPCHECK(hipMemcpyAsync(snap + nth, d.lp, nch * 8, hipMemcpyDeviceToDevice, stream));
-        HIPCHECK(hipMemcpyAsync(snap + nth + nch, d.ll, nch * 8, hipMemcpyDeviceToDevice, stream));
-        if (nph) {
-            HIPCHECK(hipMemcpyAsync(snap + nth + 2 * nch, d.phi, nph * 8, hipMemcpyDeviceToDevice, stream));
-            HIPCHECK(hipMemcpyAsync(snap + nth + 2 * nch + nph, d.phi_lp, npc * 8, hipMemcpyDeviceToDevice, stream));
-            HIPCHECK(hipMemcpyAsync(snap + nth + 2 * nch + nph + npc, d.phi_ll, npc * 8, hipMemcpyDeviceToDevice, stream));
-        }
-        HIPCHECK(hipMemcpyAsync(&ev_snap, d.evals, 8, hipMemcpyDeviceToHost, stream));
+        copy(snap, d.theta, nth * 8);
+        copy(snap + nth, d.lp, nch * 8);
+        copy(snap + nth + nch, d.ll, nch * 8);
+        copy(snap + nth + 2 * nch, d.phi, nph * 8);
+        copy(snap + nth + 2 * nch + nph, d.phi_lp, npc * 8);
+        copy(snap + nth + 2 * nch + nph + npc, d.phi_ll, npc * 8);
+        copy(ev_snap, d.evals, nev * 8);
         HIPCHECK(hipStreamSynchronize(stream));
     };
     auto restore = [&]() {
-        HIPCHECK(hipMemcpyAsync(d.theta, snap, nth * 8, hipMemcpyDeviceToDevice, stream));
-        HIPCHECK(hipMemcpyAsync(d.lp, snap + nth, nch * 8, hipMemcpyDeviceToDevice, stream));
-        HIPCHECK(hipMemcpyAsync(d.ll, snap + nth + nch, nch * 8, hipMemcpyDeviceToDevice, stream));
-        if (nph) {
-            HIPCHECK(hipMemcpyAsync(d.phi, snap + nth + 2 * nch, nph * 8, hipMemcpyDeviceToDevice, stream));
-            HIPCHECK(hipMemcpyAsync(d.phi_lp, snap + nth + 2 * nch + nph, npc * 8, hipMemcpyDeviceToDevice, stream));
-            HIPCHECK(hipMemcpyAsync(d.phi_ll, snap + nth + 2 * nch + nph + npc, npc * 8, hipMemcpyDeviceToDevice, stream));
-        }
-        HIPCHECK(hipMemcpyAsync(d.evals, &ev_snap, 8, hipMemcpyHostToDevice, stream));
+        copy(d.theta, snap, nth * 8);
+        copy(d.lp, snap + nth, nch * 8);
+        copy(d.ll, snap + nth + nch, nch * 8);
+        copy(d.phi, snap + nth + 2 * nch, nph * 8);
+        copy(d.phi_lp, snap + nth + 2 * nch + nph, npc * 8);
+        copy(d.phi_ll, snap + nth + 2 * nch + nph + npc, npc * 8);
+        copy(d.evals, ev_snap, nev * 8);
         HIPCHECK(hipStreamSynchronize(stream));
     };
     save();
     const uint32_t it0 = next_iter;
     const uint32_t ready0 = phi_ready;
     float best = 1e30f;
-    int best_t = 0, best_l = 2;
-    /* time graph replays, as production runs them */
+    Cand pick = cands[0];
     const int K = 4;
-    for (int t : cands) {
-        set_tiles(t);
-        if (d.n_tiles > 65536) continue;
-        for (int l : {1, 2}) {
-            tpl = l;
-            next_iter = it0;
-            capture(K);
-            run_batched(K); /* warm-up */
-            HIPCHECK(hipEventRecord(ev0, stream));
-            run_batched(2 * K);
-            HIPCHECK(hipEventRecord(ev1, stream));
-            HIPCHECK(hipEventSynchronize(ev1));
-            float ms = 0.f;
-            HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
-            drop_graph();
-            restore();
-            phi_ready = ready0;
-            if (ms < best) {
-                best = ms;
-                best_t = t;
-                best_l = l;
-            }
+    for (const Cand &c : cands) {
+        set_decomp(c.fused, c.k, c.tpt);
+        next_iter = it0;
+        capture(K);
+        run_batched(K); /* warm-up */
+        HIPCHECK(hipEventRecord(ev0, stream));
+        run_batched(2 * K);
+        HIPCHECK(hipEventRecord(ev1, stream));
+        HIPCHECK(hipEventSynchronize(ev1));
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
+        drop_graph();
+        restore();
+        phi_ready = ready0;
+        if (ms < best) {
+            best = ms;
+            pick = c;
         }
     }
     next_iter = it0;
-    tpl = best_l;
-    set_tiles(best_t);
+    set_decomp(pick.fused, pick.k, pick.tpt);
 }
 
 static int fail(const std::exception &e) {
@@ -752,20 +811,26 @@ int ggdmc_session_elapsed_ms(ggdmc_session *s, float *ms) {
     return GGDMC_OK;
 }
 
-int ggdmc_session_tile(ggdmc_session *s, int32_t *trials_per_tile, int32_t *n_tiles) {
+int ggdmc_session_decomposition(ggdmc_session *s, int32_t *chains_per_item,
+                                int32_t *trials_per_tile, int32_t *items_per_launch) {
     if (!s) return GGDMC_EINVAL;
+    if (chains_per_item) *chains_per_item = s->fused ? -1 : s->kchain;
     if (trials_per_tile) *trials_per_tile = s->tpt;
-    if (n_tiles) *n_tiles = s->d.n_tiles;
+    if (items_per_launch)
+        *items_per_launch = s->list_n.empty() ? 0 : *std::max_element(s->list_n.begin(), s->list_n.end());
     return GGDMC_OK;
 }
 
 int ggdmc_session_trial_evals(ggdmc_session *s, double *evals) {
     if (!s || !evals) return GGDMC_EINVAL;
     try {
-        unsigned long long v = 0;
-        HIPCHECK(hipMemcpyAsync(&v, s->d.evals, 8, hipMemcpyDeviceToHost, s->stream));
+        const size_t ns = (size_t)std::max(s->d.S, 1);
+        std::vector<unsigned long long> v(ns);
+        HIPCHECK(hipMemcpyAsync(v.data(), s->d.evals, 8 * ns, hipMemcpyDeviceToHost, s->stream));
         HIPCHECK(hipStreamSynchronize(s->stream));
-        *evals = (double)v;
+        double tot = 0.0;
+        for (unsigned long long e : v) tot += (double)e;
+        *evals = tot;
     } catch (const std::exception &e) {
         return fail(e);
     }
@@ -973,9 +1038,10 @@ int ggdmc_loglike(const ggdmc_model *model, const ggdmc_subject *data, const dou
         d.tcode = reinterpret_cast<const uint16_t *>(b + o_cell);
         d.subj_off = reinterpret_cast<const int64_t *>(b + o_off);
         d.S = 1;
-        const size_t lds = ggdmc_internal_sweep_lds(1, P, 1, nq, na, 1);
+        const int cmax = (int)std::max<int64_t>(1, std::min<int64_t>((T + kChunk - 1) / kChunk, 1024));
+        const size_t lds = ggdmc_internal_sweep_lds(1, P, 1, nq * na, 1, cmax);
         HIPCHECK(ggdmc_internal_launch_loglike(&d, reinterpret_cast<const double *>(b + o_th), n, 0,
-                                               precision, lds, nullptr,
+                                               precision, cmax, lds, nullptr,
                                                reinterpret_cast<double *>(b + o_out)));
         HIPCHECK(hipMemcpy(ll_out, b + o_out, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost));
         HIPCHECK(hipFree(dbuf));

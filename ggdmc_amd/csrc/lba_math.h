@@ -29,17 +29,20 @@
  *      * the small tail of Phi is always the one computed, so
  *        Phi(z1) - Phi(z2) is taken as Q(z2) - Q(z1) when z2 >= 0;
  *      * the survivor of a loser is formed from whichever of the untruncated
- *        CDF G or its complement den - G is small (both written without
+ *        CDF G or its complement 1 - G is small (both written without
  *        cancellation, see below) and then passed through the reference's own
- *        FP64 operations F = G/den, 1 - max(F, 1e-10). Where the reference's
- *        1 - F is quantised by FP64 rounding (survivors of 1e-16 .. 1e-12 of
- *        sharp accumulators), this reproduces the quantisation; the DBL_EPSILON
- *        replacement and its sign rule follow from the same FP64 values.
+ *        FP64 operations G = 1 + (G - 1), F = G/den, 1 - max(F, 1e-10). Where
+ *        the reference's survivor is quantised by FP64 rounding (survivors of
+ *        1e-16 .. 1e-10 of sharp accumulators, whose F sits within a few ulps
+ *        of 1), this reproduces the rounding itself, not just its scale; the
+ *        DBL_EPSILON replacement and its sign rule follow from the same FP64
+ *        values. Phi follows R's pnorm: for z > 0 it is 1 - Q(z) rounded in
+ *        FP64, Q the accurate upper tail.
  *    Untruncated CDF and complement, with Q = 1 - Phi, Pc = Phi(-v/s):
- *        G       = (zs*(phi2 - phi1) + chi*Q1 - (chi - A)*Q2) / A
- *        den - G = (Phi2 - Pc) + (chi*(Phi1 - Phi2) + zs*(phi1 - phi2)) / A
- *    (the second from integrating the drift density over the start point:
- *    P(0 < drift < (b - a)/t) averaged over a ~ U(0, A)).
+ *        G     = (zs*(phi2 - phi1) + chi*Q1 - (chi - A)*Q2) / A
+ *        1 - G = (Phi2 - Pc) + (chi*(Phi1 - Phi2) + zs*(phi1 - phi2)) / A + Pc
+ *    (the first two terms from integrating the drift density over the start
+ *    point: P(0 < drift < (b - a)/t) averaged over a ~ U(0, A)).
  *    log(x) = e*ln2 + log2(m)*ln2 with the FP32 log2 applied to the mantissa
  *    only, so its error is absolute ~1e-8 whatever the magnitude of log(x).
  */
@@ -90,14 +93,34 @@ GGDMC_FN double rcp64(double x) {
     return r * (2.0 - x * r);
 }
 
+/* 2^y for y <= 0 in FP64 arithmetic: y = n + f with n = rint(y), |f| <= 1/2,
+ * and 2^f = exp(f ln 2) from its Taylor series to degree 13 (truncation below
+ * 1e-17 for |f ln 2| <= 0.347). Underflows to 0 below 2^-1074; NaN propagates. */
+GGDMC_FN double exp2_neg(double y) {
+    if (!(y > -1075.0)) return y == y ? 0.0 : y;
+    const double n = std::rint(y);
+    const double x = (y - n) * kLn2;
+    double p = 1.6059043836821613e-10;                 /* 1/13! */
+    p = p * x + 2.08767569878681e-09;                  /* 1/12! */
+    p = p * x + 2.505210838544172e-08;                 /* 1/11! */
+    p = p * x + 2.755731922398589e-07;                 /* 1/10! */
+    p = p * x + 2.7557319223985893e-06;                /* 1/9!  */
+    p = p * x + 2.48015873015873e-05;                  /* 1/8!  */
+    p = p * x + 0.0001984126984126984;                 /* 1/7!  */
+    p = p * x + 0.001388888888888889;                  /* 1/6!  */
+    p = p * x + 0.008333333333333333;                  /* 1/5!  */
+    p = p * x + 0.041666666666666664;                  /* 1/4!  */
+    p = p * x + 0.16666666666666666;                   /* 1/3!  */
+    p = p * x + 0.5;
+    p = p * x + 1.0;
+    p = p * x + 1.0;
+    return std::ldexp(p, (int)n);
+}
+
 /* e = exp(-z^2/2), qa = Q(|z|) = upper normal tail beyond |z|. */
 template <class Ops>
 GGDMC_FN void mixed_tail(double z, double &e, double &qa) {
-    const double y = -0.5 * z * z * kLog2e;
-    const float yh = (float)y;
-    float yl = (float)(y - (double)yh);
-    if (!(yh > -1e30f)) yl = 0.0f; /* y = -inf or NaN: keep e = 0 / NaN */
-    e = (double)Ops::exp2(yh) * (1.0 + (double)yl * kLn2);
+    e = exp2_neg(-0.5 * z * z * kLog2e);
     const double a = std::fabs(z);
     const double t = rcp64<Ops>(1.0 + kMillsK * a);
     double h = kMills[kMillsDeg];
@@ -121,9 +144,10 @@ GGDMC_FN double mixed_log(double x) {
     return ((double)Ops::log2(mf) + (double)ex) * kLn2 + r;
 }
 
-/* Shared core: Phi differences and the z's of one accumulator at time t > 0. */
+/* Shared core: Phi differences and the z's of one accumulator at time t > 0;
+ * q1, q2, e1, e2 are the computed tails and exp(-z^2/2) (error bounds). */
 struct Core {
-    double zs, chi, dP, dphi, P2, Q1, Q2;
+    double zs, chi, z1, z2, dP, dphi, P2, Q1, Q2, q1, q2, e1, e2;
 };
 
 template <class Ops>
@@ -134,9 +158,15 @@ GGDMC_FN Core mixed_core(const AccK &k, double t) {
     c.chi = k.b - t * k.v;
     const double z1 = c.chi * r;
     const double z2 = (c.chi - k.A) * r;
+    c.z1 = z1;
+    c.z2 = z2;
     double e1, q1, e2, q2;
     mixed_tail<Ops>(z1, e1, q1);
     mixed_tail<Ops>(z2, e2, q2);
+    c.q1 = q1;
+    c.q2 = q2;
+    c.e1 = e1;
+    c.e2 = e2;
     c.Q1 = z1 >= 0.0 ? q1 : 1.0 - q1;
     const double P1 = z1 >= 0.0 ? 1.0 - q1 : q1;
     c.Q2 = z2 >= 0.0 ? q2 : 1.0 - q2;
@@ -144,36 +174,6 @@ GGDMC_FN Core mixed_core(const AccK &k, double t) {
     c.dP = z2 >= 0.0 ? c.Q2 - c.Q1 : P1 - c.P2; /* Phi(z1) - Phi(z2) */
     c.dphi = (e2 - e1) * kInvSqrt2Pi;            /* phi(z2) - phi(z1) */
     return c;
-}
-
-/* Winner: log(max(pdf, 1e-10)); NaN propagates. */
-template <class Ops>
-GGDMC_FN double mixed_winner_logpdf(const AccK &k, double rt) {
-    const double t = rt - k.t0;
-    if (!(t > 0.0)) return t == t ? kLogFloor : t;
-    const Core c = mixed_core<Ops>(k, t);
-    const double pdf = (k.v * c.dP + k.s * c.dphi) * k.invA * k.inv_den;
-    return pdf < kFloor ? kLogFloor : mixed_log<Ops>(pdf);
-}
-
-/* Loser: the reference's FP64 survivor 1 - max(F, 1e-10) (may be <= 0). */
-template <class Ops>
-GGDMC_FN double mixed_loser_survivor(const AccK &k, double rt) {
-    const double t = rt - k.t0;
-    if (!(t > 0.0)) return t == t ? 1.0 - kFloor : t;
-    const Core c = mixed_core<Ops>(k, t);
-    const double G = (c.zs * c.dphi + c.chi * c.Q1 - (c.chi - k.A) * c.Q2) * k.invA;
-    const double Su = (c.P2 - k.Pc) + (c.chi * c.dP - c.zs * c.dphi) * k.invA;
-    const double Gd = (G * k.inv_den < 0.5) ? G : (k.den - Su);
-    const double F = Gd * k.inv_den;
-    return 1.0 - (F > kFloor ? F : (F == F ? kFloor : F));
-}
-
-/* Accumulates one survivor factor into (log |prod|, sign, zero) state. */
-GGDMC_FN void fold_survivor(double S, double logS, double &acc, int &neg, bool &zero) {
-    if (S == 0.0) zero = true;
-    if (S < 0.0) neg ^= 1;
-    acc += logS;
 }
 
 /* ------------------------------------------------------------- naive FP64 */
@@ -193,6 +193,112 @@ GGDMC_FN void naive_pdf_cdf(const AccK &k, double t, double &pdf, double &cdf) {
     pdf = (k.v * (P1 - P2) + k.s * (d2 - d1)) / (k.A * k.den);
     cdf = (1.0 + (zs * (d2 - d1) + chimA * P2 - chi * P1) / k.A) / k.den;
 }
+
+/* Error model of the mixed evaluation: every computed tail and exp(-z^2/2)
+ * carries a relative error of up to kTailErr (FP32 exp2); a result whose
+ * bound exceeds kCondTol relative, or whose reference FP64 value carries more
+ * rounding noise than that (Phi rounded near 1, ulp 1.1e-16, scaled by the
+ * formula's coefficients), is flagged ill-conditioned and the trial is
+ * evaluated with the reference's FP64 formula instead. */
+constexpr double kTailErr = 1e-9;   /* Mills-ratio fit (8.2e-10) */
+constexpr double kExpErr = 1e-15;   /* exp(-z^2/2) in FP64 arithmetic */
+constexpr double kCondTol = 1e-5;
+constexpr double kUlp1 = 1.1102230246251565e-16;
+
+/* Winner: log(max(pdf, 1e-10)); NaN propagates. */
+template <class Ops>
+GGDMC_FN double mixed_winner_logpdf(const AccK &k, double rt, bool &ill) {
+    const double t = rt - k.t0;
+    if (!(t > 0.0)) return t == t ? kLogFloor : t;
+    const Core c = mixed_core<Ops>(k, t);
+    const double num = k.v * c.dP + k.s * c.dphi;
+    const double err = kTailErr * std::fabs(k.v) * (c.q1 + c.q2) + kExpErr * k.s * (c.e1 + c.e2) * kInvSqrt2Pi +
+                       (c.z2 > 0.0 ? 2.0 * kUlp1 * std::fabs(k.v) : 0.0);
+    const double pdf = num * k.invA * k.inv_den;
+    /* both floored unless the pdf or its uncertainty reaches the floor */
+    if (err > kCondTol * std::fabs(num) && (std::fabs(num) + err) * k.invA * k.inv_den > 0.5 * kFloor)
+        ill = true;
+    return pdf < kFloor ? kLogFloor : mixed_log<Ops>(pdf);
+}
+
+/* Loser: the reference's FP64 survivor 1 - max(F, 1e-10) (may be <= 0). */
+template <class Ops>
+GGDMC_FN double mixed_loser_survivor(const AccK &k, double rt, bool &ill) {
+    const double t = rt - k.t0;
+    if (!(t > 0.0)) return t == t ? 1.0 - kFloor : t;
+    const Core c = mixed_core<Ops>(k, t);
+    const double achi = std::fabs(c.chi), achiA = std::fabs(c.chi - k.A);
+    const double G = (c.zs * c.dphi + c.chi * c.Q1 - (c.chi - k.A) * c.Q2) * k.invA;
+    double F;
+    if (G * k.inv_den < 0.5) {
+        F = G * k.inv_den;
+    } else {
+        /* 1 - G without cancellation, then the reference's own FP64 steps
+         * F = (1 + (G - 1)) / den: the rounding of 1 + (G - 1) is what
+         * quantises the reference's survivor, and it is reproduced exactly
+         * whenever 1 - G is accurate to well under half an ulp of 1. */
+        const double omG = (c.P2 - k.Pc) + (c.chi * c.dP - c.zs * c.dphi) * k.invA + k.Pc;
+        F = (1.0 - omG) / k.den;
+    }
+    const double S = 1.0 - (F > kFloor ? F : (F == F ? kFloor : F));
+    /* error of F from the tails, plus the reference's own noise from Phi
+     * values rounded near 1 (z > 0), against the survivor */
+    const double errF = (kExpErr * c.zs * (c.e1 + c.e2) * kInvSqrt2Pi + kTailErr * (achi * c.q1 + achiA * c.q2) +
+                         kUlp1 * ((c.z1 > 0.0 ? achi : 0.0) + (c.z2 > 0.0 ? achiA : 0.0))) *
+                        k.invA * k.inv_den;
+    /* below a hundredth of an ulp of 1 the reference's own rounding of F is
+     * reproduced exactly, whatever the survivor */
+    if (errF > kCondTol * std::fabs(S) && errF > 1e-18) ill = true;
+    return S;
+}
+
+/* Accumulates one survivor factor into (log |prod|, sign, zero) state. */
+GGDMC_FN void fold_survivor(double S, double logS, double &acc, int &neg, bool &zero) {
+    if (S == 0.0) zero = true;
+    if (S < 0.0) neg ^= 1;
+    acc += logS;
+}
+
+/* Reference FP64 trial term (src formula above), for the plain path and for
+ * the batched path's ill-conditioned trials. */
+template <class Ops>
+GGDMC_FN double naive_trial_term(const AccK *kc, int n_acc, int w, double rt) {
+    double pdf, cdf;
+    naive_pdf_cdf<Ops>(kc[w], rt - kc[w].t0, pdf, cdf);
+    double dens = pdf != pdf ? pdf : (pdf > kFloor ? pdf : kFloor);
+    for (int j = 0; j < n_acc; ++j) {
+        if (j == w) continue;
+        naive_pdf_cdf<Ops>(kc[j], rt - kc[j].t0, pdf, cdf);
+        const double F = cdf != cdf ? cdf : (cdf > kFloor ? cdf : kFloor);
+        dens *= (1.0 - F);
+    }
+    return std::log(dens <= 0.0 ? 2.220446049250313e-16 : dens);
+}
+
+/* Batched-path trial term: mixed evaluation, or the reference formula when
+ * any accumulator's evaluation is ill-conditioned. */
+template <class Ops>
+GGDMC_FN double mixed_trial_term(const AccK *kc, int n_acc, int w, double rt) {
+    bool ill = false;
+    const double lp = mixed_winner_logpdf<Ops>(kc[w], rt, ill);
+    double acc = 0.0, nanv = lp;
+    int neg = 0;
+    bool zero = false, nan = lp != lp;
+    for (int j = 0; j < n_acc; ++j) {
+        if (j == w) continue;
+        const double S = mixed_loser_survivor<Ops>(kc[j], rt, ill);
+        if (S != S) {
+            nan = true;
+            nanv = S;
+        }
+        const double aS = std::fabs(S);
+        fold_survivor(S, aS > 0.0 ? mixed_log<Ops>(aS) : 0.0, acc, neg, zero);
+    }
+    if (nan) return nanv;
+    if (ill) return naive_trial_term<Ops>(kc, n_acc, w, rt);
+    return (zero || neg) ? kLogDblEps : lp + acc;
+}
+
 
 /* ----------------------------------------------------------- per (cell, acc)
  * Resolves the six families from theta / constants and fills AccK.

@@ -100,8 +100,12 @@ GGDMC_RFN double u32_open(uint32_t w) { return ((double)w + 0.5) * 2.32830643653
 /* floor(w * n / 2^32): uniform on 0..n-1. */
 GGDMC_RFN uint32_t below(uint32_t w, uint32_t n) { return (uint32_t)(((uint64_t)w * n) >> 32); }
 
+/* Jitter of a perturbed parameter from its word: U(-rp, rp). */
+GGDMC_RFN double jitter_word(uint32_t w, double rp) { return rp * (2.0 * u32_open(w) - 1.0); }
+
+/* Jitter k of a slot: word 4 + k, i.e. word k & 3 of block 1 + k / 4. */
 GGDMC_RFN double jitter(const Stream &st, uint32_t slot, uint32_t k, double rp) {
-    return rp * (2.0 * u32_open(st.word(slot, 4 + k)) - 1.0);
+    return jitter_word(st.word(slot, 4 + k), rp);
 }
 
 GGDMC_RFN double decision_u(const Stream &st) {

@@ -1,15 +1,17 @@
 /*
  * HIP kernels of the DE-MCMC sampler (gfx950, wave64).
  *
- *  subject_sweep_kernel  DE crossover of one chain group (or all groups) of every
- *                        subject: proposal (src/de.cpp:567-613 / :111-155),
- *                        LBA log-likelihood over a trial tile, last-arriving
- *                        tile accepts (src/de.cpp:81-108); migration ring
+ *  subject_sweep_kernel  one workgroup per work item (subject, chain range of a
+ *                        chain group, trial tile): DE crossover proposals
+ *                        (src/de.cpp:567-613 / :111-155), LBA log-likelihood as
+ *                        canonical chunk sums, Metropolis accept (:81-108) by the
+ *                        item itself or by the last-arriving tile; migration ring
  *                        (src/de.cpp:615-665 / :157-199) by the subject's first
- *                        tile.
- *  phi_kernel            per-chain sufficient statistics of the subject states,
- *                        then the population-level step (src/de.cpp:385-465,
- *                        :467-565) with sumloghlike (:245-270) from them.
+ *                        item; thinned store by the subject's last item.
+ *  phi_kernel            per-(chain, parameter) sufficient statistics of the
+ *                        subject states, then the population-level step
+ *                        (src/de.cpp:385-465, :467-565) with sumloghlike
+ *                        (:245-270) from them.
  *  plain_*_kernel        the reference arithmetic and sequential order: one
  *                        launch per chain step (the FP64 comparator path).
  *  loglike_kernel        sumloglike of given parameter vectors.
@@ -28,7 +30,10 @@ struct DevOps {
     static __device__ __forceinline__ float log2(float x) { return __builtin_amdgcn_logf(x); }
     static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
     static __device__ __forceinline__ double erfc(double x) { return ::erfc(x); }
-    static __device__ __forceinline__ double pnorm(double z) { return 0.5 * ::erfc(-z * kSqrt1_2); }
+    /* R's pnorm: the lower tail accurately, above 0 as 1 - upper tail */
+    static __device__ __forceinline__ double pnorm(double z) {
+        return z > 0.0 ? 1.0 - 0.5 * ::erfc(z * kSqrt1_2) : 0.5 * ::erfc(-z * kSqrt1_2);
+    }
     static __device__ __forceinline__ double dnorm(double z) {
         return kInvSqrt2Pi * ::exp(-0.5 * z * z);
     }
@@ -41,32 +46,8 @@ constexpr int kAccKDoubles = sizeof(AccK) / sizeof(double);
 /* --------------------------------------------------------------- likelihood */
 template <int MODE>
 __device__ __forceinline__ double trial_term(const AccK *kc, int n_acc, int w, double rt) {
-    if (MODE == MODE_MIXED) {
-        const double lp = mixed_winner_logpdf<DevOps>(kc[w], rt);
-        double acc = 0.0;
-        int neg = 0;
-        bool zero = false, nan = lp != lp;
-        for (int j = 0; j < n_acc; ++j) {
-            if (j == w) continue;
-            const double S = mixed_loser_survivor<DevOps>(kc[j], rt);
-            nan |= S != S;
-            const double aS = fabs(S);
-            fold_survivor(S, aS > 0.0 ? mixed_log<DevOps>(aS) : 0.0, acc, neg, zero);
-        }
-        if (nan) return __builtin_nan("");
-        return (zero || neg) ? kLogDblEps : lp + acc;
-    } else {
-        double pdf, cdf;
-        naive_pdf_cdf<DevOps>(kc[w], rt - kc[w].t0, pdf, cdf);
-        double dens = pdf != pdf ? pdf : (pdf > kFloor ? pdf : kFloor);
-        for (int j = 0; j < n_acc; ++j) {
-            if (j == w) continue;
-            naive_pdf_cdf<DevOps>(kc[j], rt - kc[j].t0, pdf, cdf);
-            const double F = cdf != cdf ? cdf : (cdf > kFloor ? cdf : kFloor);
-            dens *= (1.0 - F);
-        }
-        return ::log(dens <= 0.0 ? 2.220446049250313e-16 : dens);
-    }
+    return MODE == MODE_MIXED ? mixed_trial_term<DevOps>(kc, n_acc, w, rt)
+                              : naive_trial_term<DevOps>(kc, n_acc, w, rt);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -88,47 +69,37 @@ __device__ double block_sum(double v, double *red) {
     return s;
 }
 
-/* Evaluates trials [t_lo, t_hi) for chains [0, kb) of a batch whose AccK
- * table is acck[kb][n_cond][n_acc]; adds each wave's per-chain sums to
- * part[wave][cb + c]. Units (chain, block of 64*TPL trials) are split into
- * contiguous ranges per wave, so a wave reduces once per chain it touches. */
-template <int MODE, int TPL>
-__device__ void eval_units(const Dev &d, int64_t t_lo, int64_t t_hi, int kb, int cb,
-                           const int *active, const double *acck, double *part, int ng_stride) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int nw = blockDim.x >> 6;
-    const int64_t span = 64 * TPL;
-    const int nblk = (int)((t_hi - t_lo + span - 1) / span);
-    const int U = kb * nblk;
-    const int u0 = (int)(((int64_t)wave * U) / nw), u1 = (int)(((int64_t)(wave + 1) * U) / nw);
+/* Canonical chunk sums: C[c * c_stride + jj] = sum of the trial terms of chunk
+ * j_lo + jj of the subject whose trials are [t0, t_end), for the kb chains
+ * whose constants are acck[c][n_cond][n_acc]. A wave evaluates one
+ * (chain, chunk) unit at a time: lane l adds the terms of trials
+ * t0 + j*kChunk + l and + 64 + l (zero past the end), then the fixed xor
+ * butterfly. Chains with active[c] == 0 (prior density zero) get 0. */
+template <int MODE>
+__device__ void chunk_sums(const Dev &d, int64_t t0, int64_t t_end, int j_lo, int nj, int kb,
+                           const int *active, const double *acck, double *C, int c_stride) {
+    static_assert(kChunk == 128, "two trials per lane");
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int per_chain = d.n_cond * d.n_acc;
-    int cur = -1;
-    double acc = 0.0;
-    for (int u = u0; u < u1; ++u) {
-        const int c = u / nblk, blk = u - c * nblk;
-        if (c != cur) {
-            if (cur >= 0) {
-                const double v = wave_sum(acc);
-                if (lane == 0) part[wave * ng_stride + cb + cur] += v;
+    for (int u = wave; u < kb * nj; u += nw) {
+        const int c = u / nj, jj = u - c * nj;
+        double v = 0.0;
+        if (active[c]) {
+            const AccK *kc = reinterpret_cast<const AccK *>(acck) + (size_t)c * per_chain;
+            const int64_t i1 = t0 + (int64_t)(j_lo + jj) * kChunk + lane, i2 = i1 + 64;
+            double a = 0.0, b = 0.0;
+            if (i1 < t_end) {
+                const int code = d.tcode[i1];
+                a = trial_term<MODE>(kc + (code >> 3) * d.n_acc, d.n_acc, code & 7, d.rt[i1]);
             }
-            cur = c;
-            acc = 0.0;
-        }
-        if (!active[cb + c]) continue;
-        const AccK *kc = reinterpret_cast<const AccK *>(acck) + (size_t)c * per_chain;
-        const int64_t base = t_lo + (int64_t)blk * span + lane;
-#pragma unroll
-        for (int t = 0; t < TPL; ++t) {
-            const int64_t i = base + 64 * t;
-            if (i < t_hi) {
-                const int code = d.tcode[i];
-                acc += trial_term<MODE>(kc + (code >> 3) * d.n_acc, d.n_acc, code & 7, d.rt[i]);
+            if (i2 < t_end) {
+                const int code = d.tcode[i2];
+                b = trial_term<MODE>(kc + (code >> 3) * d.n_acc, d.n_acc, code & 7, d.rt[i2]);
             }
+            v = a + b;
         }
-    }
-    if (cur >= 0) {
-        const double v = wave_sum(acc);
-        if (lane == 0) part[wave * ng_stride + cb + cur] += v;
+        v = wave_sum(v);
+        if (lane == 0) C[(size_t)c * c_stride + jj] = v;
     }
 }
 
@@ -154,22 +125,24 @@ __device__ double log_prior1(int dist, double p0, double p1, double lo, double u
     return __builtin_nan("");
 }
 
-/* Subject-level log prior of x for chain `chain`: the run's fixed prior, or
- * for hierarchical runs tnorm(loc = phi head, scale = phi tail) (src/de.cpp:599-600). */
-__device__ double subject_logprior(const Dev &d, const JobDev &job, int chain, const double *x) {
-    const int P = d.P;
-    double s = 0.0;
+/* Parameter p's term of the subject-level log prior for chain `chain`: the
+ * run's fixed prior, or for hierarchical runs tnorm(loc = phi head,
+ * scale = phi tail) (src/de.cpp:599-600). */
+__device__ __forceinline__ double subject_prior_term(const Dev &d, const JobDev &job, int chain, int p,
+                                                     double x) {
     if (job.kind == GGDMC_KIND_HIER) {
-        const double *ph = d.phi + ((size_t)job.phi_index * d.nchain + chain) * (2 * P);
-        for (int p = 0; p < P; ++p)
-            s += log_prior1(GGDMC_DIST_TNORM, ph[p], ph[P + p], d.pr_lo[job.prior_off + p],
-                            d.pr_up[job.prior_off + p], x[p]);
-    } else {
-        for (int p = 0; p < P; ++p) {
-            const int k = job.prior_off + p;
-            s += log_prior1(d.pr_dist[k], d.pr_p0[k], d.pr_p1[k], d.pr_lo[k], d.pr_up[k], x[p]);
-        }
+        const double *ph = d.phi + ((size_t)job.phi_index * d.nchain + chain) * (2 * d.P);
+        return log_prior1(GGDMC_DIST_TNORM, ph[p], ph[d.P + p], d.pr_lo[job.prior_off + p],
+                          d.pr_up[job.prior_off + p], x);
     }
+    const int k = job.prior_off + p;
+    return log_prior1(d.pr_dist[k], d.pr_p0[k], d.pr_p1[k], d.pr_lo[k], d.pr_up[k], x);
+}
+
+/* sumlogprior (src/de.cpp:144, :186) of x: terms added in parameter order. */
+__device__ double subject_logprior(const Dev &d, const JobDev &job, int chain, const double *x) {
+    double s = 0.0;
+    for (int p = 0; p < d.P; ++p) s += subject_prior_term(d, job, chain, p, x[p]);
     return s;
 }
 
@@ -181,39 +154,49 @@ __device__ __forceinline__ bool mh_accept(double tpost, double cur, double U) {
 
 /* -------------------------------------------------------- LDS carve-up */
 struct SweepLds {
-    double *th, *lp, *ll, *prop, *tlp, *U, *part, *tile, *acck, *red;
-    int *active, *sub, *flag;
+    double *th, *lp, *ll;   /* [n*P], [n], [n]: the subject's chain states */
+    double *prop, *tm;      /* [kmax*P]: proposals, their prior terms */
+    double *tlp, *U, *tll;  /* [kmax] */
+    double *acck;           /* [chain_batch * per_chain] accumulator constants */
+    double *csum;           /* [kmax * cmax] chunk sums */
+    int *m0, *m1, *active, *acc; /* [kmax] */
+    int *sub;               /* [n] migration subset */
+    int *flag;              /* [4] */
 };
 
-__host__ __device__ inline size_t sweep_lds_bytes(int n, int P, int ng, int n_cond, int n_acc,
-                                                  int chain_batch) {
-    size_t dbl = (size_t)n * P + 2 * n + (size_t)ng * P + 3 * ng + (size_t)kWaves * ng +
-                 (size_t)chain_batch * n_cond * n_acc * kAccKDoubles + kWaves;
-    size_t ints = (size_t)ng + n + 4;
+__host__ __device__ inline size_t sweep_lds_bytes(int n, int P, int kmax, int per_chain,
+                                                  int chain_batch, int cmax) {
+    const size_t dbl = (size_t)n * P + 2 * (size_t)n + 2 * (size_t)kmax * P + 3 * (size_t)kmax +
+                       (size_t)chain_batch * per_chain * kAccKDoubles + (size_t)kmax * cmax;
+    const size_t ints = 4 * (size_t)kmax + n + 4;
     return dbl * sizeof(double) + ints * sizeof(int);
 }
 
-__device__ SweepLds carve(char *base, int n, int P, int ng, int n_cond, int n_acc, int chain_batch) {
+__device__ SweepLds carve(char *base, int n, int P, int kmax, int per_chain, int chain_batch,
+                          int cmax) {
     SweepLds L;
     double *p = reinterpret_cast<double *>(base);
     L.th = p; p += (size_t)n * P;
     L.lp = p; p += n;
     L.ll = p; p += n;
-    L.prop = p; p += (size_t)ng * P;
-    L.tlp = p; p += ng;
-    L.U = p; p += ng;
-    L.tile = p; p += ng;
-    L.part = p; p += (size_t)kWaves * ng;
-    L.acck = p; p += (size_t)chain_batch * n_cond * n_acc * kAccKDoubles;
-    L.red = p; p += kWaves;
+    L.prop = p; p += (size_t)kmax * P;
+    L.tm = p; p += (size_t)kmax * P;
+    L.tlp = p; p += kmax;
+    L.U = p; p += kmax;
+    L.tll = p; p += kmax;
+    L.acck = p; p += (size_t)chain_batch * per_chain * kAccKDoubles;
+    L.csum = p; p += (size_t)kmax * cmax;
     int *q = reinterpret_cast<int *>(p);
-    L.active = q; q += ng;
+    L.m0 = q; q += kmax;
+    L.m1 = q; q += kmax;
+    L.active = q; q += kmax;
+    L.acc = q; q += kmax;
     L.sub = q; q += n;
     L.flag = q;
     return L;
 }
 
-/* Fills the AccK tables of chains [cb, cb+kb) of the proposal batch. */
+/* Fills the accumulator constants of proposals [cb, cb+kb). */
 __device__ void build_acck(const Dev &d, const double *prop, int P, int cb, int kb, double *acck) {
     const int per_chain = d.n_cond * d.n_acc;
     AccK *K = reinterpret_cast<AccK *>(acck);
@@ -223,246 +206,342 @@ __device__ void build_acck(const Dev &d, const double *prop, int P, int cb, int 
     }
 }
 
-/* Likelihood of proposals prop[0..ng) over trials [t_lo, t_hi); result per
- * chain in L.tile (summed over waves in fixed order). */
-template <int MODE, int TPL>
-__device__ void tile_loglike(const Dev &d, const SweepLds &L, int ng, int chain_batch,
-                             int64_t t_lo, int64_t t_hi) {
+/* Prior terms of proposals L.prop[0..k) for chains i0 + c, one thread per
+ * (chain, parameter), summed per chain in parameter order into L.tlp. */
+__device__ void proposal_priors(const Dev &d, const JobDev &job, const SweepLds &L, int i0, int k) {
     const int P = d.P;
-    for (int e = threadIdx.x; e < kWaves * ng; e += blockDim.x) L.part[e] = 0.0;
-    for (int cb = 0; cb < ng; cb += chain_batch) {
-        const int kb = min(chain_batch, ng - cb);
+    for (int e = threadIdx.x; e < k * P; e += blockDim.x) {
+        const int c = e / P, p = e - c * P;
+        L.tm[e] = subject_prior_term(d, job, i0 + c, p, L.prop[e]);
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < k; c += blockDim.x) {
+        double s = 0.0;
+        for (int p = 0; p < P; ++p) s += L.tm[c * P + p];
+        L.tlp[c] = s;
+        L.active[c] = s > -__builtin_inf();
+    }
+    __syncthreads();
+}
+
+/* DE proposals of chains i0 .. i0+k-1 of group [lo, hi) (src/de.cpp:583-597,
+ * :119-135): partners and accept U per chain, the jitter of parameters
+ * 4b..4b+3 from Philox block 1 + b per (chain, b), prior terms per
+ * (chain, parameter). para >= 0: blocked update of that parameter only. */
+__device__ void crossover_proposals(const Dev &d, const JobDev &job, const SweepLds &L,
+                                    const Stream &st, int lo, int hi, int i0, int k, int para) {
+    const int P = d.P, n = d.nchain;
+    for (int c = threadIdx.x; c < k; c += blockDim.x) {
+        int m0, m1;
+        double U;
+        partners(st, i0 + c, lo, hi, n, m0, m1, U);
+        L.m0[c] = m0;
+        L.m1[c] = m1;
+        L.U[c] = U;
+    }
+    __syncthreads();
+    const int nb = para >= 0 ? 1 : (P + 3) >> 2;
+    for (int e = threadIdx.x; e < k * nb; e += blockDim.x) {
+        const int c = e / nb, b = e - c * nb;
+        const int i = i0 + c;
+        const double *xi = L.th + (size_t)i * P;
+        const double *x0 = L.th + (size_t)L.m0[c] * P, *x1 = L.th + (size_t)L.m1[c] * P;
+        double *x = L.prop + (size_t)c * P;
+        const U4 r = st.block((uint32_t)i, 1u + (uint32_t)b);
+        if (para >= 0) {
+            for (int p = 0; p < P; ++p) x[p] = xi[p];
+            x[para] += jitter_word(r.x[0], d.rp) + job.gamma * (x0[para] - x1[para]);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int p = 4 * b + q;
+                if (p < P) x[p] = xi[p] + (jitter_word(r.x[q], d.rp) + job.gamma * (x0[p] - x1[p]));
+            }
+        }
+    }
+    __syncthreads();
+    proposal_priors(d, job, L, i0, k);
+}
+
+/* Log-likelihood of proposals L.prop[0..k) over chunks [j_lo, j_lo + nj) of
+ * the subject's trials [t0, t_end). to_global: each chunk sum goes to
+ * d.csum[(cbase + j) * n + i0 + c]; otherwise their sequential sum goes to
+ * L.tll[c]. */
+template <int MODE>
+__device__ void proposals_loglike(const Dev &d, const SweepLds &L, int k, int chain_batch,
+                                  int cmax, int64_t t0, int64_t t_end, int j_lo, int nj,
+                                  bool to_global, int64_t cbase, int i0) {
+    const int P = d.P, n = d.nchain;
+    for (int c = threadIdx.x; c < k; c += blockDim.x) L.tll[c] = 0.0;
+    for (int cb = 0; cb < k; cb += chain_batch) {
+        const int kb = min(chain_batch, k - cb);
         __syncthreads();
         build_acck(d, L.prop, P, cb, kb, L.acck);
         __syncthreads();
-        eval_units<MODE, TPL>(d, t_lo, t_hi, kb, cb, L.active, L.acck, L.part, ng);
+        for (int jb = 0; jb < nj; jb += cmax) {
+            const int nbat = min(cmax, nj - jb);
+            chunk_sums<MODE>(d, t0, t_end, j_lo + jb, nbat, kb, L.active + cb, L.acck, L.csum, cmax);
+            __syncthreads();
+            if (to_global) {
+                for (int e = threadIdx.x; e < kb * nbat; e += blockDim.x) {
+                    const int c = e / nbat, jj = e - c * nbat;
+                    d.csum[(size_t)(cbase + j_lo + jb + jj) * n + i0 + cb + c] = L.csum[(size_t)c * cmax + jj];
+                }
+            } else {
+                for (int c = threadIdx.x; c < kb; c += blockDim.x) {
+                    double s = L.tll[cb + c];
+                    for (int jj = 0; jj < nbat; ++jj) s += L.csum[(size_t)c * cmax + jj];
+                    L.tll[cb + c] = s;
+                }
+            }
+            __syncthreads();
+        }
     }
-    __syncthreads();
-    for (int c = threadIdx.x; c < ng; c += blockDim.x) {
-        double v = 0.0;
-        for (int w = 0; w < kWaves; ++w) v += L.part[w * ng + c];
-        L.tile[c] = v;
-    }
-    __syncthreads();
 }
 
-__device__ void store_subject(const Dev &d, const SweepLds &L, int s, uint32_t iter) {
-    if (iter % (uint32_t)d.thin != 0) return;
-    const int k = (int)(iter / (uint32_t)d.thin);
-    if (k >= d.nmc) return;
+/* Metropolis accept of proposals c < k for chains i0 + c (src/de.cpp:81-108):
+ * state updated in LDS and in global memory. */
+__device__ void accept_proposals(const Dev &d, const SweepLds &L, int x, int i0, int k) {
     const int n = d.nchain, P = d.P;
-    double *ot = d.out_theta + ((size_t)s * d.nmc + k) * n * P;
-    for (int e = threadIdx.x; e < n * P; e += blockDim.x) ot[e] = L.th[e];
-    for (int c = threadIdx.x; c < n; c += blockDim.x) {
-        d.out_lp[((size_t)s * d.nmc + k) * n + c] = L.lp[c];
-        d.out_ll[((size_t)s * d.nmc + k) * n + c] = L.ll[c];
+    for (int c = threadIdx.x; c < k; c += blockDim.x) {
+        const int i = i0 + c;
+        const double cur = L.lp[i] + L.ll[i];
+        const bool a = mh_accept(L.tlp[c] + L.tll[c], cur, L.U[c]);
+        L.acc[c] = a;
+        if (a) {
+            L.lp[i] = L.tlp[c];
+            L.ll[i] = L.tll[c];
+            d.lp[(size_t)x * n + i] = L.tlp[c];
+            d.ll[(size_t)x * n + i] = L.tll[c];
+        }
     }
+    __syncthreads();
+    for (int e = threadIdx.x; e < k * P; e += blockDim.x) {
+        const int c = e / P;
+        if (!L.acc[c]) continue;
+        const size_t r = (size_t)(i0 + c) * P + (e - c * P);
+        L.th[r] = L.prop[e];
+        d.theta[(size_t)x * n * P + r] = L.prop[e];
+    }
+    __syncthreads();
 }
 
-/* Migration ring of subject s over all its trials (src/de.cpp:615-665,
- * :157-199). Runs in one workgroup; state in L is updated in place and in
- * global memory. */
-template <int MODE, int TPL>
-__device__ void subject_migration(const Dev &d, const SweepLds &L, const JobDev &job, int s,
-                                  const Stream &st, int para, int chain_batch) {
+/* Migration ring of subject x over all its trials (src/de.cpp:615-665,
+ * :157-199), in one workgroup; state updated in LDS and global memory. */
+template <int MODE>
+__device__ void subject_migration(const Dev &d, const SweepLds &L, const JobDev &job, int x,
+                                  const Stream &st, int para, int cmax) {
     const int n = d.nchain, P = d.P;
     if (threadIdx.x == 0) L.flag[0] = subchains(st, n, L.sub);
     __syncthreads();
     const int ns = L.flag[0];
-    const int64_t t_lo = d.subj_off[s], t_hi = d.subj_off[s + 1];
+    const int64_t t0 = d.subj_off[x], t1 = d.subj_off[x + 1];
+    const int nj = (int)((t1 - t0 + kChunk - 1) / kChunk);
     for (int k = 0; k < ns; ++k) {
         const int src = L.sub[k], dst = L.sub[(k + 1) % ns];
         const uint32_t slot = kSlotMigrate | (uint32_t)k;
-        if (threadIdx.x == 0) {
-            double *x = L.prop;
-            for (int p = 0; p < P; ++p) x[p] = L.th[src * P + p];
+        const int nb = para >= 0 ? 1 : (P + 3) >> 2;
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+            const U4 r = st.block(slot, 1u + (uint32_t)b);
+            const double *xs = L.th + (size_t)src * P;
             if (para >= 0) {
-                x[para] += jitter(st, slot, 0, d.rp);
+                for (int p = 0; p < P; ++p) L.prop[p] = xs[p];
+                L.prop[para] += jitter_word(r.x[0], d.rp);
             } else {
-                for (int p = 0; p < P; ++p) x[p] += jitter(st, slot, p, d.rp);
-            }
-            const U4 b = st.block(slot, 0);
-            L.U[0] = u53(b.x[0], b.x[1]);
-            const double tlp = subject_logprior(d, job, src, x);
-            L.tlp[0] = tlp;
-            L.active[0] = tlp > -__builtin_inf();
-        }
-        __syncthreads();
-        tile_loglike<MODE, TPL>(d, L, 1, chain_batch, t_lo, t_hi);
-        if (threadIdx.x == 0) {
-            const double tll = L.tile[0];
-            const double cur = L.lp[dst] + L.ll[dst];
-            if (L.active[0]) atomicAdd(d.evals, (unsigned long long)(t_hi - t_lo));
-            if (mh_accept(L.tlp[0] + tll, cur, L.U[0])) {
-                for (int p = 0; p < P; ++p) {
-                    L.th[dst * P + p] = L.prop[p];
-                    d.theta[((size_t)s * n + dst) * P + p] = L.prop[p];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int p = 4 * b + q;
+                    if (p < P) L.prop[p] = xs[p] + jitter_word(r.x[q], d.rp);
                 }
+            }
+        }
+        if (threadIdx.x == 0) {
+            const U4 b0 = st.block(slot, 0);
+            L.U[0] = u53(b0.x[0], b0.x[1]);
+        }
+        __syncthreads();
+        proposal_priors(d, job, L, src, 1);
+        proposals_loglike<MODE>(d, L, 1, 1, cmax, t0, t1, 0, nj, false, 0, 0);
+        if (threadIdx.x == 0) {
+            if (L.active[0]) atomicAdd(d.evals + x, (unsigned long long)(t1 - t0));
+            const double cur = L.lp[dst] + L.ll[dst];
+            const bool a = mh_accept(L.tlp[0] + L.tll[0], cur, L.U[0]);
+            L.flag[1] = a;
+            if (a) {
                 L.lp[dst] = L.tlp[0];
-                L.ll[dst] = tll;
-                d.lp[(size_t)s * n + dst] = L.tlp[0];
-                d.ll[(size_t)s * n + dst] = tll;
+                L.ll[dst] = L.tll[0];
+                d.lp[(size_t)x * n + dst] = L.tlp[0];
+                d.ll[(size_t)x * n + dst] = L.tll[0];
             }
         }
         __syncthreads();
+        if (L.flag[1]) {
+            for (int p = threadIdx.x; p < P; p += blockDim.x) {
+                L.th[(size_t)dst * P + p] = L.prop[p];
+                d.theta[((size_t)x * n + dst) * P + p] = L.prop[p];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+/* Thinned store of subject x's state (src: LDS copy or global state). */
+__device__ void store_subject(const Dev &d, const double *th, const double *lp, const double *ll,
+                              int x, uint32_t iter) {
+    if (iter % (uint32_t)d.thin != 0) return;
+    const int k = (int)(iter / (uint32_t)d.thin);
+    if (k >= d.nmc) return;
+    const int n = d.nchain, P = d.P;
+    double *ot = d.out_theta + ((size_t)x * d.nmc + k) * n * P;
+    for (int e = threadIdx.x; e < n * P; e += blockDim.x) ot[e] = th[e];
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+        d.out_lp[((size_t)x * d.nmc + k) * n + c] = lp[c];
+        d.out_ll[((size_t)x * d.nmc + k) * n + c] = ll[c];
     }
 }
 
 /* ------------------------------------------------------------ sweep kernel */
-__device__ void fused_phi_tail(const Dev &d, int s, const JobDev &job, uint32_t iter,
-                               char *lds_raw);
+__device__ void fused_phi_tail(const Dev &d, int x, const JobDev &job, uint32_t iter,
+                               char *lds_raw, int *flag);
 
-template <int MODE, int TPL>
+/* End of subject x's iteration in the final launch: the last of its w items
+ * stores the state (re-read from global memory when other items wrote it)
+ * and, for a hierarchical run, joins the run's population-step countdown. */
+__device__ void finish_subject(const Dev &d, const SweepLds &L, const JobDev &job, int x, int w,
+                               uint32_t iter, bool fuse_phi, char *lds_raw) {
+    if (w > 1) {
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t old = atomicAdd(d.subj_ctr + x, 1u);
+            L.flag[2] = old == (uint32_t)(w - 1);
+        }
+        __syncthreads();
+        if (!L.flag[2]) return;
+        __threadfence();
+        if (threadIdx.x == 0) d.subj_ctr[x] = 0u;
+        const size_t nP = (size_t)d.nchain * d.P;
+        store_subject(d, d.theta + x * nP, d.lp + (size_t)x * d.nchain, d.ll + (size_t)x * d.nchain, x, iter);
+    } else {
+        store_subject(d, L.th, L.lp, L.ll, x, iter);
+    }
+    if (fuse_phi && job.kind == GGDMC_KIND_HIER) fused_phi_tail(d, x, job, iter, lds_raw, L.flag);
+}
+
+template <int MODE>
 __global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs a) {
     extern __shared__ __attribute__((aligned(16))) char lds_raw[];
-    const int tile = blockIdx.x;
-    const int s = d.tile_subj[tile];
-    const JobDev job = d.jobs[d.subj_job[s]];
+    const int4 item = a.items[blockIdx.x];
+    const int x = item.x;
+    const int c0 = item.y >> 16, c1 = item.y & 0xFFFF;
+    const int t = item.z >> 16, ntile = item.z & 0xFFFF;
+    const int w = item.w;
+    const JobDev job = d.jobs[d.subj_job[x]];
     const int n = d.nchain, P = d.P, G = d.G;
-    const int ng_max = (n + G - 1) / G + 1;
-    const SweepLds L = carve(lds_raw, n, P, ng_max, d.n_cond, d.n_acc, a.chain_batch);
+    const SweepLds L = carve(lds_raw, n, P, a.kmax, d.n_cond * d.n_acc, a.chain_batch, a.cmax);
     const uint32_t iter = *d.iter_base + a.iter_off;
     const bool single = job.kind != GGDMC_KIND_HIER;
     const bool blocked = d.blocked_sub != 0;
-    const int ntile = d.subj_ntile[s];
-    const bool first_tile = tile == d.subj_tile0[s];
+    const uint64_t seed = ((uint64_t)job.k1 << 32) | job.k0;
+    const uint32_t unit = (uint32_t)d.subj_unit[x];
 
     /* level decision (src/de.cpp:208-236 single level, :333-372 subjects) */
     const int dsweep = (blocked && single) ? 0 : a.sweep;
-    const Stream sd = make_stream(((uint64_t)job.k1 << 32) | job.k0, iter + d.philox_offset, (uint32_t)d.subj_unit[s],
-                                  (uint32_t)dsweep);
+    const Stream sd = make_stream(seed, iter + d.philox_offset, unit, (uint32_t)dsweep);
     const bool migrate = decision_u(sd) < d.sub_mig;
 
-    for (int e = threadIdx.x; e < n * P; e += blockDim.x) L.th[e] = d.theta[(size_t)s * n * P + e];
+    for (int e = threadIdx.x; e < n * P; e += blockDim.x) L.th[e] = d.theta[(size_t)x * n * P + e];
     for (int c = threadIdx.x; c < n; c += blockDim.x) {
-        L.lp[c] = d.lp[(size_t)s * n + c];
-        L.ll[c] = d.ll[(size_t)s * n + c];
+        L.lp[c] = d.lp[(size_t)x * n + c];
+        L.ll[c] = d.ll[(size_t)x * n + c];
     }
     __syncthreads();
 
     if (migrate) {
-        if (!first_tile) return;
         const bool act = blocked && single ? a.sweep == 0 : true;
-        if (act && a.g_begin == 0) {
+        if (c0 == 0 && t == 0 && act && a.g_begin == 0) {
             const int para = (blocked && !single) ? a.sweep : -1;
-            subject_migration<MODE, TPL>(d, L, job, s, sd, para, a.chain_batch);
+            subject_migration<MODE>(d, L, job, x, sd, para, a.cmax);
         }
-        if (a.final_launch) {
-            store_subject(d, L, s, iter);
-            if (a.fuse_phi && job.kind == GGDMC_KIND_HIER) fused_phi_tail(d, s, job, iter, lds_raw);
-        }
-        return;
-    }
-
-    const int para = blocked ? a.sweep : -1;
-    const Stream st = make_stream(((uint64_t)job.k1 << 32) | job.k0, iter + d.philox_offset, (uint32_t)d.subj_unit[s],
-                                  (uint32_t)a.sweep);
-    const int64_t t_lo = d.tile_lo[tile], t_hi = d.tile_hi[tile];
-    const int64_t ntr = d.subj_off[s + 1] - d.subj_off[s];
-
-    for (int g = a.g_begin; g < a.g_end; ++g) {
-        const int lo = group_lo(g, n, G), hi = group_lo(g + 1, n, G);
-        const int ng = hi - lo;
-        /* proposals (src/de.cpp:583-597) */
-        for (int c = threadIdx.x; c < ng; c += blockDim.x) {
-            const int i = lo + c;
-            int m0, m1;
-            double U;
-            partners(st, i, lo, hi, n, m0, m1, U);
-            L.U[c] = U;
-            double *x = L.prop + (size_t)c * P;
-            const double *xi = L.th + (size_t)i * P;
-            const double *x0 = L.th + (size_t)m0 * P, *x1 = L.th + (size_t)m1 * P;
-            for (int p = 0; p < P; ++p) x[p] = xi[p];
-            if (para >= 0) {
-                x[para] += jitter(st, (uint32_t)i, 0, d.rp) + job.gamma * (x0[para] - x1[para]);
-            } else {
-                for (int p = 0; p < P; ++p)
-                    x[p] += jitter(st, (uint32_t)i, p, d.rp) + job.gamma * (x0[p] - x1[p]);
-            }
-            const double tlp = subject_logprior(d, job, i, x);
-            L.tlp[c] = tlp;
-            L.active[c] = tlp > -__builtin_inf();
-        }
-        __syncthreads();
-        tile_loglike<MODE, TPL>(d, L, ng, a.chain_batch, t_lo, t_hi);
-
-        bool last = true;
-        if (ntile > 1) {
-            for (int c = threadIdx.x; c < ng; c += blockDim.x)
-                d.partial[(size_t)tile * n + lo + c] = L.tile[c];
-            __threadfence();
-            __syncthreads();
+    } else {
+        const int para = blocked ? a.sweep : -1;
+        const Stream st = make_stream(seed, iter + d.philox_offset, unit, (uint32_t)a.sweep);
+        const int64_t s0 = d.subj_off[x], s1 = d.subj_off[x + 1];
+        const int64_t t_lo = ntile > 1 ? s0 + (int64_t)t * d.tpt : s0;
+        const int64_t t_hi = ntile > 1 ? min(s1, t_lo + (int64_t)d.tpt) : s1;
+        const int j_lo = (int)((t_lo - s0) / kChunk);
+        const int nj = (int)((t_hi - t_lo + kChunk - 1) / kChunk);
+        for (int g = a.g_begin; g < a.g_end; ++g) {
+            const int lo = group_lo(g, n, G), hi = group_lo(g + 1, n, G);
+            const int k = min(c1 == kAllChains ? hi - lo : c1, hi - lo) - c0;
+            if (k <= 0) continue;
+            const int i0 = lo + c0;
+            crossover_proposals(d, job, L, st, lo, hi, i0, k, para);
+            proposals_loglike<MODE>(d, L, k, a.chain_batch, a.cmax, s0, s1, j_lo, nj, ntile > 1,
+                                    d.chunk_off[x], i0);
             if (threadIdx.x == 0) {
-                const uint32_t old = atomicAdd(d.counter + s, 1u);
-                L.flag[1] = old == (uint32_t)(ntile - 1);
+                int na = 0;
+                for (int c = 0; c < k; ++c) na += L.active[c];
+                atomicAdd(d.evals + x, (unsigned long long)na * (unsigned long long)(t_hi - t_lo));
             }
-            __syncthreads();
-            last = L.flag[1] != 0;
-            if (!last) return;
-            __threadfence();
-        }
-        /* accept (src/de.cpp:81-108) by the last tile of the subject */
-        for (int c = threadIdx.x; c < ng; c += blockDim.x) {
-            const int i = lo + c;
-            double tll = L.tile[c];
             if (ntile > 1) {
-                tll = 0.0;
-                const int t0 = d.subj_tile0[s];
-                for (int t = 0; t < ntile; ++t) tll += d.partial[(size_t)(t0 + t) * n + i];
-            }
-            const double cur = L.lp[i] + L.ll[i];
-            if (mh_accept(L.tlp[c] + tll, cur, L.U[c])) {
-                for (int p = 0; p < P; ++p) {
-                    const double v = L.prop[(size_t)c * P + p];
-                    L.th[(size_t)i * P + p] = v;
-                    d.theta[((size_t)s * n + i) * P + p] = v;
+                /* the last-arriving tile of (subject, chain range) accepts */
+                __threadfence();
+                __syncthreads();
+                uint32_t *ctr = d.tile_ctr + (blockIdx.x - t);
+                if (threadIdx.x == 0) {
+                    const uint32_t old = atomicAdd(ctr, 1u);
+                    L.flag[3] = old == (uint32_t)(ntile - 1);
                 }
-                L.lp[i] = L.tlp[c];
-                L.ll[i] = tll;
-                d.lp[(size_t)s * n + i] = L.tlp[c];
-                d.ll[(size_t)s * n + i] = tll;
+                __syncthreads();
+                if (!L.flag[3]) continue;
+                __threadfence();
+                if (threadIdx.x == 0) *ctr = 0u;
+                const int64_t cb = d.chunk_off[x], ncb = d.chunk_off[x + 1] - cb;
+                for (int c = threadIdx.x; c < k; c += blockDim.x) {
+                    double s = 0.0;
+                    for (int64_t j = 0; j < ncb; ++j) s += d.csum[(size_t)(cb + j) * n + i0 + c];
+                    L.tll[c] = s;
+                }
+                __syncthreads();
             }
+            accept_proposals(d, L, x, i0, k);
         }
-        if (threadIdx.x == 0) {
-            int na = 0;
-            for (int c = 0; c < ng; ++c) na += L.active[c];
-            atomicAdd(d.evals, (unsigned long long)na * (unsigned long long)ntr);
-            if (ntile > 1) d.counter[s] = 0u;
-        }
-        __syncthreads();
     }
-    if (a.final_launch) {
-        store_subject(d, L, s, iter);
-        if (a.fuse_phi && job.kind == GGDMC_KIND_HIER) fused_phi_tail(d, s, job, iter, lds_raw);
-    }
+    if (a.final_launch) finish_subject(d, L, job, x, w, iter, a.fuse_phi != 0, lds_raw);
 }
 
 /* ------------------------------------------------------------- phi level */
 struct PhiLds {
-    double *ph, *plp, *pll, *st, *prop, *U, *tlp, *red, *tm;
-    int *sub, *flag;
+    double *ph, *plp, *pll;  /* [n*2P], [n], [n]: population states */
+    double *prop;            /* [ngmax*2P] proposals of a chain group */
+    double *tm;              /* [(ngmax+1)*4P] prior and likelihood terms */
+    double *U;               /* [ngmax] */
+    int *m0, *m1, *acc;      /* [ngmax] */
+    int *sub;                /* [n] */
+    int *flag;               /* [4] */
 };
 
-__host__ __device__ inline size_t phi_lds_bytes(int n, int P) {
-    const size_t dbl = (size_t)n * 2 * P + 2 * n + (size_t)n * P * 4 + (size_t)n * 2 * P + 2 * n +
-                       kWaves + (size_t)2 * n * P;
-    return dbl * sizeof(double) + (size_t)(n + 4) * sizeof(int);
+__host__ __device__ inline size_t phi_lds_bytes(int n, int P, int G) {
+    const size_t ng = (size_t)(n + G - 1) / G;
+    const size_t dbl = (size_t)n * 2 * P + 2 * (size_t)n + ng * 2 * P + (ng + 1) * 4 * P + ng;
+    return dbl * sizeof(double) + (3 * ng + n + 4) * sizeof(int);
 }
 
-__device__ PhiLds phi_carve(char *base, int n, int P) {
+__device__ PhiLds phi_carve(char *base, int n, int P, int G) {
+    const int ng = (n + G - 1) / G;
     PhiLds L;
     double *p = reinterpret_cast<double *>(base);
     L.ph = p; p += (size_t)n * 2 * P;
     L.plp = p; p += n;
     L.pll = p; p += n;
-    L.st = p; p += (size_t)n * P * 4;
-    L.prop = p; p += (size_t)n * 2 * P;
-    L.U = p; p += n;
-    L.tlp = p; p += n;
-    L.red = p; p += kWaves;
-    L.tm = p; p += (size_t)2 * n * P;
+    L.prop = p; p += (size_t)ng * 2 * P;
+    L.tm = p; p += (size_t)(ng + 1) * 4 * P;
+    L.U = p; p += ng;
     int *q = reinterpret_cast<int *>(p);
+    L.m0 = q; q += ng;
+    L.m1 = q; q += ng;
+    L.acc = q; q += ng;
     L.sub = q; q += n;
     L.flag = q;
     return L;
@@ -473,7 +552,8 @@ __device__ PhiLds phi_carve(char *base, int n, int P) {
  *   sum_s log tnorm(x_s; mu, sg, lo, up)
  *     = -(M2 + S (mean - mu)^2) / (2 sg^2) - S (log sqrt(2 pi) + log sg + log Z)
  * NaN for sg <= 0 or NaN inputs, -inf when a subject value is outside
- * [lo, up]; summing the P shares reproduces the per-subject sum's NaN / -inf. */
+ * [lo, up]; summing the P shares in parameter order gives sumloghlike
+ * (src/de.cpp:245-270). */
 __device__ double hll_term(const Dev &d, const JobDev &job, int p, double mu, double sg,
                            const double *st, int S) {
     const double mean = st[0], M2 = st[1], mn = st[2], mx = st[3];
@@ -486,190 +566,247 @@ __device__ double hll_term(const Dev &d, const JobDev &job, int p, double mu, do
            (double)S * (0.918938533204672741780329736406 + ::log(sg) + ::log(Z));
 }
 
-/* sumloghlike (src/de.cpp:245-270) from per-parameter sufficient statistics
- * of the S subject values x_s: sum_s log tnorm(x_s; mu, sg, lo, up)
- *   = -(M2 + S (mean - mu)^2) / (2 sg^2) - S (log sqrt(2 pi) + log sg + log Z). */
-__device__ double hyper_ll(const Dev &d, const JobDev &job, const double *phi, const double *st,
-                           int S) {
-    double out = 0.0;
-    for (int p = 0; p < d.P; ++p) out += hll_term(d, job, p, phi[p], phi[d.P + p], st + p * 4, S);
-    return out;
+__device__ __forceinline__ double hyper_prior_term(const Dev &d, const JobDev &job, int q, double x) {
+    const int k = job.hprior_off + q;
+    return log_prior1(d.pr_dist[k], d.pr_p0[k], d.pr_p1[k], d.pr_lo[k], d.pr_up[k], x);
 }
 
+/* sumlogprior of a population state x (src/de.cpp:426, :518): 2P terms in order. */
 __device__ double hyper_logprior(const Dev &d, const JobDev &job, const double *x) {
     double s = 0.0;
-    for (int p = 0; p < 2 * d.P; ++p) {
-        const int k = job.hprior_off + p;
-        s += log_prior1(d.pr_dist[k], d.pr_p0[k], d.pr_p1[k], d.pr_lo[k], d.pr_up[k], x[p]);
-    }
+    for (int q = 0; q < 2 * d.P; ++q) s += hyper_prior_term(d, job, q, x[q]);
     return s;
 }
 
-/* Sufficient statistics of chain c over the run's subjects, two passes;
- * each group of 16 lanes reduces one parameter with a fixed shuffle tree. */
-__device__ void chain_stats_range(const Dev &d, const JobDev &job, int c0, int c1, double *out) {
+/* Sufficient statistics (mean, M2, min, max) of pairs q in [q0, q1)
+ * (q = chain * P + parameter) over the run's S subjects: one thread per pair,
+ * subjects in order, two passes. */
+__device__ void chain_stats_pairs(const Dev &d, const JobDev &job, int q0, int q1, double *out) {
     const int P = d.P, n = d.nchain, S = job.nsubject;
-    const int l16 = threadIdx.x & 15, ngrp = blockDim.x >> 4;
-    for (int q = threadIdx.x >> 4; q < (c1 - c0) * P; q += ngrp) {
-        const int c = c0 + q / P, p = q - (q / P) * P;
+    const size_t stride = (size_t)n * P;
+    for (int q = q0 + (int)threadIdx.x; q < q1; q += blockDim.x) {
+        const double *src = d.theta + (size_t)job.first_subject * stride + q;
         double sum = 0.0, mn = __builtin_inf(), mx = -__builtin_inf();
-        int nan = 0;
-        for (int s = l16; s < S; s += 16) {
-            const double x = d.theta[((size_t)(job.first_subject + s) * n + c) * P + p];
+        bool nan = false;
+#pragma unroll 8
+        for (int s = 0; s < S; ++s) {
+            const double x = src[s * stride];
             sum += x;
             mn = fmin(mn, x);
             mx = fmax(mx, x);
             nan |= x != x;
         }
-#pragma unroll
-        for (int off = 8; off > 0; off >>= 1) {
-            sum += __shfl_xor(sum, off, 16);
-            mn = fmin(mn, __shfl_xor(mn, off, 16));
-            mx = fmax(mx, __shfl_xor(mx, off, 16));
-            nan |= __shfl_xor(nan, off, 16);
-        }
         const double mean = sum / (double)S;
         double m2 = 0.0;
-        for (int s = l16; s < S; s += 16) {
-            const double x = d.theta[((size_t)(job.first_subject + s) * n + c) * P + p];
-            m2 += (x - mean) * (x - mean);
+#pragma unroll 8
+        for (int s = 0; s < S; ++s) {
+            const double dx = src[s * stride] - mean;
+            m2 += dx * dx;
         }
-#pragma unroll
-        for (int off = 8; off > 0; off >>= 1) m2 += __shfl_xor(m2, off, 16);
-        if (l16 == 0) {
-            double *o = out + ((size_t)(c - c0) * P + p) * 4;
-            o[0] = nan ? __builtin_nan("") : mean;
-            o[1] = m2;
-            o[2] = mn;
-            o[3] = mx;
-        }
+        double *o = out + (size_t)q * 4;
+        o[0] = nan ? __builtin_nan("") : mean;
+        o[1] = m2;
+        o[2] = mn;
+        o[3] = mx;
     }
 }
 
-__device__ void chain_stats(const Dev &d, const JobDev &job, int c, double *out) {
-    chain_stats_range(d, job, c, c + 1, out);
+/* Migration ring of population state f (src/de.cpp:467-565): per ring step the
+ * proposal, its prior and likelihood terms, and the refreshed likelihoods of
+ * source and target (hierarchical runs, src/de.cpp:494-500) in parallel, the
+ * sums and the accept by one thread. */
+__device__ void phi_migration(const Dev &d, const JobDev &job, const PhiLds &L, const Stream &sd,
+                              int para, const double *stats, bool hier) {
+    const int n = d.nchain, P = d.P, P2 = 2 * P, S = job.nsubject;
+    if (threadIdx.x == 0) L.flag[0] = subchains(sd, n, L.sub);
+    __syncthreads();
+    const int ns = L.flag[0];
+    for (int k = 0; k < ns; ++k) {
+        const int src = L.sub[k], dst = L.sub[(k + 1) % ns];
+        const uint32_t slot = kSlotMigrate | (uint32_t)k;
+        const double *xs = L.ph + (size_t)src * P2;
+        const int nb = para >= 0 ? 1 : (P2 + 3) >> 2;
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+            const U4 r = sd.block(slot, 1u + (uint32_t)b);
+            if (para >= 0) {
+                for (int q = 0; q < P2; ++q) L.prop[q] = xs[q];
+                L.prop[para] += jitter_word(r.x[0], d.rp);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int q = 4 * b + j;
+                    if (q < P2) L.prop[q] = xs[q] + jitter_word(r.x[j], d.rp);
+                }
+            }
+        }
+        __syncthreads();
+        /* tm: [0, 2P) prior of the proposal, [2P, 3P) its likelihood shares,
+         * [3P, 4P) and [4P, 5P) the refreshed shares of src and dst */
+        const int nterm = hier ? 5 * P : 3 * P;
+        const double *st_src = stats + (size_t)(hier ? src : 0) * P * 4;
+        for (int e = threadIdx.x; e < nterm; e += blockDim.x) {
+            double v;
+            if (e < P2) {
+                v = hyper_prior_term(d, job, e, L.prop[e]);
+            } else if (e < 3 * P) {
+                const int p = e - P2;
+                v = hll_term(d, job, p, L.prop[p], L.prop[P + p], st_src + p * 4, S);
+            } else {
+                const int which = e < 4 * P ? src : dst;
+                const int p = e - (e < 4 * P ? 3 * P : 4 * P);
+                const double *ph = L.ph + (size_t)which * P2;
+                v = hll_term(d, job, p, ph[p], ph[P + p], stats + ((size_t)which * P + p) * 4, S);
+            }
+            L.tm[e] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (hier) {
+                double a = 0.0, b = 0.0;
+                for (int p = 0; p < P; ++p) a += L.tm[3 * P + p];
+                for (int p = 0; p < P; ++p) b += L.tm[4 * P + p];
+                L.pll[src] = a;
+                L.pll[dst] = b;
+            }
+            double tlp = 0.0, tll = 0.0;
+            for (int q = 0; q < P2; ++q) tlp += L.tm[q];
+            for (int p = 0; p < P; ++p) tll += L.tm[P2 + p];
+            const double cur = L.plp[dst] + L.pll[dst];
+            const U4 b0 = sd.block(slot, 0);
+            const bool a = mh_accept(tlp + tll, cur, u53(b0.x[0], b0.x[1]));
+            L.flag[1] = a;
+            if (a) {
+                L.plp[dst] = tlp;
+                L.pll[dst] = tll;
+            }
+        }
+        __syncthreads();
+        if (L.flag[1])
+            for (int q = threadIdx.x; q < P2; q += blockDim.x) L.ph[(size_t)dst * P2 + q] = L.prop[q];
+        __syncthreads();
+    }
 }
 
-/* Population step of one run in one workgroup: phi state in LDS, chain i
- * handled by thread i. */
-__device__ void phi_update(const Dev &d, int f, const JobDev &job, uint32_t iter, bool store,
-                           const PhiLds &L) {
+/* Crossover of chain group [lo, hi) of a population state (src/de.cpp:385-465):
+ * partners per chain, jitter per (chain, Philox block), prior terms per
+ * (chain, parameter of 2P), likelihood shares per (chain, parameter) for the
+ * proposal and the refreshed current state (src/de.cpp:397), sums per chain. */
+__device__ void phi_crossover(const Dev &d, const JobDev &job, const PhiLds &L, const Stream &st,
+                              int lo, int hi, int para, const double *stats, bool hier) {
+    const int n = d.nchain, P = d.P, P2 = 2 * P, S = job.nsubject, ng = hi - lo;
+    for (int c = threadIdx.x; c < ng; c += blockDim.x) {
+        int m0, m1;
+        double U;
+        partners(st, lo + c, lo, hi, n, m0, m1, U);
+        L.m0[c] = m0;
+        L.m1[c] = m1;
+        L.U[c] = U;
+    }
+    __syncthreads();
+    const int nb = para >= 0 ? 1 : (P2 + 3) >> 2;
+    for (int e = threadIdx.x; e < ng * nb; e += blockDim.x) {
+        const int c = e / nb, b = e - c * nb;
+        const int i = lo + c;
+        const double *xi = L.ph + (size_t)i * P2;
+        const double *x0 = L.ph + (size_t)L.m0[c] * P2, *x1 = L.ph + (size_t)L.m1[c] * P2;
+        double *x = L.prop + (size_t)c * P2;
+        const U4 r = st.block((uint32_t)i, 1u + (uint32_t)b);
+        if (para >= 0) {
+            for (int q = 0; q < P2; ++q) x[q] = xi[q];
+            x[para] += jitter_word(r.x[0], d.rp) + job.gamma * (x0[para] - x1[para]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int q = 4 * b + j;
+                if (q < P2) x[q] = xi[q] + (jitter_word(r.x[j], d.rp) + job.gamma * (x0[q] - x1[q]));
+            }
+        }
+    }
+    __syncthreads();
+    /* per chain c, tm[c*4P + ...]: [0, 2P) prior, [2P, 3P) proposal shares,
+     * [3P, 4P) refreshed current shares */
+    const int nterm = hier ? 4 * P : 3 * P;
+    for (int e = threadIdx.x; e < ng * nterm; e += blockDim.x) {
+        const int c = e / nterm, r = e - c * nterm;
+        const int i = lo + c;
+        const double *x = L.prop + (size_t)c * P2;
+        double v;
+        if (r < P2) {
+            v = hyper_prior_term(d, job, r, x[r]);
+        } else {
+            const int p = r < 3 * P ? r - P2 : r - 3 * P;
+            const double *sti = stats + ((size_t)(hier ? i : 0) * P + p) * 4;
+            const double *v2 = r < 3 * P ? x : L.ph + (size_t)i * P2;
+            v = hll_term(d, job, p, v2[p], v2[P + p], sti, S);
+        }
+        L.tm[(size_t)c * 4 * P + r] = v;
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < ng; c += blockDim.x) {
+        const int i = lo + c;
+        const double *tc = L.tm + (size_t)c * 4 * P;
+        double tlp = 0.0, tll = 0.0;
+        for (int q = 0; q < P2; ++q) tlp += tc[q];
+        for (int p = 0; p < P; ++p) tll += tc[P2 + p];
+        if (hier) {
+            double ref = 0.0;
+            for (int p = 0; p < P; ++p) ref += tc[3 * P + p];
+            L.pll[i] = ref;
+        }
+        const double cur = L.pll[i] + L.plp[i];
+        const bool a = mh_accept(tlp + tll, cur, L.U[c]);
+        L.acc[c] = a;
+        if (a) {
+            L.plp[i] = tlp;
+            L.pll[i] = tll;
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < ng * P2; e += blockDim.x) {
+        const int c = e / P2;
+        if (L.acc[c]) L.ph[(size_t)lo * P2 + e] = L.prop[e];
+    }
+    __syncthreads();
+}
+
+/* Population step of state f for iteration iter in one workgroup, from the
+ * per-chain statistics in d.stats (src/de.cpp:272-383 decisions). */
+__device__ void phi_update(const Dev &d, int f, const JobDev &job, uint32_t iter, const PhiLds &L) {
     const int n = d.nchain, P = d.P, P2 = 2 * P, G = d.G;
     const bool hier = job.kind == GGDMC_KIND_HIER;
-    const int S = job.nsubject;
+    const double *stats = d.stats + (size_t)f * n * P * 4;
     for (int e = threadIdx.x; e < n * P2; e += blockDim.x) L.ph[e] = d.phi[(size_t)f * n * P2 + e];
     for (int c = threadIdx.x; c < n; c += blockDim.x) {
         L.plp[c] = d.phi_lp[(size_t)f * n + c];
         L.pll[c] = d.phi_ll[(size_t)f * n + c];
     }
-    const int nst = hier ? n : 1;
-    for (int e = threadIdx.x; e < nst * P * 4; e += blockDim.x)
-        L.st[e] = d.stats[(size_t)f * n * P * 4 + e];
     __syncthreads();
     const uint32_t unit = hier ? kUnitPhi : 0u;
     const double mig = hier ? d.pop_mig : d.sub_mig;
     const bool blocked = hier ? d.blocked_pop != 0 : d.blocked_sub != 0;
     const uint64_t seed = ((uint64_t)job.k1 << 32) | job.k0;
     const int nsweep = blocked ? P2 : 1;
-    bool single_mig_done = false;
     for (int sw = 0; sw < nsweep; ++sw) {
-        /* run_chains (single level) decides once; run_hchains per parameter */
+        /* run_chains (single level) decides once per iteration; run_hchains
+         * per parameter */
         const int dsweep = (blocked && !hier) ? 0 : sw;
         const Stream sd = make_stream(seed, iter + d.philox_offset, unit, (uint32_t)dsweep);
-        const bool migrate = decision_u(sd) < mig;
-        if (migrate) {
-            if (!hier && blocked) {
-                if (single_mig_done || sw > 0) break;
-                single_mig_done = true;
-            }
-            const int para = (blocked && hier) ? sw : -1;
-            if (threadIdx.x == 0) {
-                const int ns = subchains(sd, n, L.sub);
-                for (int k = 0; k < ns; ++k) {
-                    const int src = L.sub[k], dst = L.sub[(k + 1) % ns];
-                    if (hier) {
-                        L.pll[src] = hyper_ll(d, job, L.ph + (size_t)src * P2, L.st + (size_t)src * P * 4, S);
-                        L.pll[dst] = hyper_ll(d, job, L.ph + (size_t)dst * P2, L.st + (size_t)dst * P * 4, S);
-                    }
-                    const uint32_t slot = kSlotMigrate | (uint32_t)k;
-                    double *x = L.prop;
-                    for (int p = 0; p < P2; ++p) x[p] = L.ph[(size_t)src * P2 + p];
-                    if (para >= 0) {
-                        x[para] += jitter(sd, slot, 0, d.rp);
-                    } else {
-                        for (int p = 0; p < P2; ++p) x[p] += jitter(sd, slot, p, d.rp);
-                    }
-                    const double tlp = hyper_logprior(d, job, x);
-                    const double tll = hyper_ll(d, job, x, L.st + (size_t)(hier ? src : 0) * P * 4, S);
-                    const double cur = L.plp[dst] + L.pll[dst];
-                    const U4 b = sd.block(slot, 0);
-                    if (mh_accept(tlp + tll, cur, u53(b.x[0], b.x[1]))) {
-                        for (int p = 0; p < P2; ++p) L.ph[(size_t)dst * P2 + p] = x[p];
-                        L.plp[dst] = tlp;
-                        L.pll[dst] = tll;
-                    }
-                }
-            }
-            __syncthreads();
+        if (decision_u(sd) < mig) {
+            phi_migration(d, job, L, sd, (blocked && hier) ? sw : -1, stats, hier);
             if (!hier && blocked) break;
             continue;
         }
-        const int para = blocked ? sw : -1;
         const Stream st = make_stream(seed, iter + d.philox_offset, unit, (uint32_t)sw);
-        for (int g = 0; g < G; ++g) {
-            const int lo = group_lo(g, n, G), hi = group_lo(g + 1, n, G), ng = hi - lo;
-            /* proposals from partners outside the group (src/de.cpp:402-423) */
-            for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-                int m0, m1;
-                double U;
-                partners(st, i, lo, hi, n, m0, m1, U);
-                L.U[i] = U;
-                double *x = L.prop + (size_t)i * P2;
-                const double *x0 = L.ph + (size_t)m0 * P2, *x1 = L.ph + (size_t)m1 * P2;
-                for (int p = 0; p < P2; ++p) x[p] = L.ph[(size_t)i * P2 + p];
-                if (para >= 0) {
-                    x[para] += jitter(st, (uint32_t)i, 0, d.rp) + job.gamma * (x0[para] - x1[para]);
-                } else {
-                    for (int p = 0; p < P2; ++p)
-                        x[p] += jitter(st, (uint32_t)i, p, d.rp) + job.gamma * (x0[p] - x1[p]);
-                }
-                L.tlp[i] = hyper_logprior(d, job, x);
-            }
-            __syncthreads();
-            /* per (chain, parameter): the refreshed current share (src/de.cpp:397)
-             * and the proposal's share of sumloghlike */
-            const int nt = ng * P;
-            for (int e = threadIdx.x; e < (hier ? 2 : 1) * nt; e += blockDim.x) {
-                const int which = e / nt, r = e - which * nt;
-                const int i = lo + r / P, p = r - (r / P) * P;
-                const double *sti = L.st + (size_t)(hier ? i : 0) * P * 4 + p * 4;
-                const double *v = which == 0 ? L.prop + (size_t)i * P2 : L.ph + (size_t)i * P2;
-                L.tm[(size_t)which * n * P + (size_t)i * P + p] = hll_term(d, job, p, v[p], v[P + p], sti, S);
-            }
-            __syncthreads();
-            for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-                double tll = 0.0, ref = 0.0;
-                for (int p = 0; p < P; ++p) {
-                    tll += L.tm[(size_t)i * P + p];
-                    if (hier) ref += L.tm[(size_t)n * P + (size_t)i * P + p];
-                }
-                if (hier) L.pll[i] = ref;
-                const double cur = L.pll[i] + L.plp[i];
-                if (mh_accept(L.tlp[i] + tll, cur, L.U[i])) {
-                    for (int p = 0; p < P2; ++p) L.ph[(size_t)i * P2 + p] = L.prop[(size_t)i * P2 + p];
-                    L.plp[i] = L.tlp[i];
-                    L.pll[i] = tll;
-                }
-            }
-            __syncthreads();
-        }
+        for (int g = 0; g < G; ++g)
+            phi_crossover(d, job, L, st, group_lo(g, n, G), group_lo(g + 1, n, G), blocked ? sw : -1,
+                          stats, hier);
     }
     for (int e = threadIdx.x; e < n * P2; e += blockDim.x) d.phi[(size_t)f * n * P2 + e] = L.ph[e];
     for (int c = threadIdx.x; c < n; c += blockDim.x) {
         d.phi_lp[(size_t)f * n + c] = L.plp[c];
         d.phi_ll[(size_t)f * n + c] = L.pll[c];
     }
-    if (store && iter % (uint32_t)d.thin == 0) {
+    if (iter % (uint32_t)d.thin == 0) {
         const int k = (int)(iter / (uint32_t)d.thin);
         if (k < d.nmc) {
             double *ot = d.phi_out_theta + ((size_t)f * d.nmc + k) * n * P2;
@@ -682,47 +819,46 @@ __device__ void phi_update(const Dev &d, int f, const JobDev &job, uint32_t iter
     }
 }
 
-/* grid = n_phi * nchain: block (f, c) computes chain c's statistics of a
- * hierarchical run; the last block of each run performs its population step.
- * Hyper-only runs have fixed statistics: their block c == 0 steps directly. */
-__global__ __launch_bounds__(kBlock) void phi_kernel(Dev d, uint32_t iter_off) {
+/* grid = n_phi * nblk: block (f, b) computes the statistics of pairs
+ * [b * blockDim, (b + 1) * blockDim) of a hierarchical run; the last block of
+ * each run performs its population step. Hyper-only runs have fixed
+ * statistics: their block b == 0 steps directly. */
+__global__ __launch_bounds__(kBlock) void phi_kernel(Dev d, uint32_t iter_off, int nblk) {
     extern __shared__ __attribute__((aligned(16))) char lds_raw[];
     const int n = d.nchain, P = d.P;
-    const int f = blockIdx.x / n, c = blockIdx.x - f * n;
+    const int f = blockIdx.x / nblk, b = blockIdx.x - f * nblk;
     const JobDev job = d.jobs[d.phi_job[f]];
     const uint32_t iter = *d.iter_base + iter_off;
-    const PhiLds L = phi_carve(lds_raw, n, P);
+    const PhiLds L = phi_carve(lds_raw, n, P, d.G);
     if (job.kind != GGDMC_KIND_HIER) {
-        if (c != 0) return;
-        phi_update(d, f, job, iter, true, L);
+        if (b != 0) return;
+        phi_update(d, f, job, iter, L);
         return;
     }
-    chain_stats(d, job, c, d.stats + ((size_t)f * n + c) * P * 4);
+    const int q0 = b * (int)blockDim.x, q1 = min(n * P, q0 + (int)blockDim.x);
+    chain_stats_pairs(d, job, q0, q1, d.stats + (size_t)f * n * P * 4);
     __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) {
-        const uint32_t old = atomicAdd(d.counter + d.S + f, 1u);
-        L.flag[0] = old == (uint32_t)(n - 1);
+        const uint32_t old = atomicAdd(d.stats_ctr + f, 1u);
+        L.flag[0] = old == (uint32_t)(nblk - 1);
     }
     __syncthreads();
     if (!L.flag[0]) return;
     __threadfence();
-    if (threadIdx.x == 0) d.counter[d.S + f] = 0u;
-    phi_update(d, f, job, iter, true, L);
+    if (threadIdx.x == 0) d.stats_ctr[f] = 0u;
+    phi_update(d, f, job, iter, L);
 }
 
-/* Called by the workgroup that finished subject s's iteration: the last
+/* Called by the workgroup that finished subject x's iteration: the last
  * subject of the run to finish computes every chain's statistics and the
- * population step of the next iteration (the run's subjects are all done, so
- * no workgroup still reads the population state). */
-__device__ void fused_phi_tail(const Dev &d, int s, const JobDev &job, uint32_t iter,
-                               char *lds_raw) {
+ * population step of the next iteration (all of the run's subjects are done,
+ * so no workgroup still reads its population state). */
+__device__ void fused_phi_tail(const Dev &d, int x, const JobDev &job, uint32_t iter,
+                               char *lds_raw, int *flag) {
     __threadfence();
     __syncthreads();
-    int *flag = reinterpret_cast<int *>(lds_raw);
-    const int jidx = d.subj_job[s];
-    uint32_t *ctr = d.counter + d.S + d.n_phi + jidx;
-    __syncthreads();
+    uint32_t *ctr = d.job_ctr + d.subj_job[x];
     if (threadIdx.x == 0) {
         const uint32_t old = atomicAdd(ctr, 1u);
         flag[0] = old == (uint32_t)(job.nsubject - 1);
@@ -735,11 +871,10 @@ __device__ void fused_phi_tail(const Dev &d, int s, const JobDev &job, uint32_t 
     if (threadIdx.x == 0) *ctr = 0u;
     if (iter + 1 > d.last_iter) return;
     const int f = job.phi_index, n = d.nchain, P = d.P;
-    chain_stats_range(d, job, 0, n, d.stats + (size_t)f * n * P * 4);
-    __threadfence();
+    chain_stats_pairs(d, job, 0, n * P, d.stats + (size_t)f * n * P * 4);
     __syncthreads();
-    const PhiLds L = phi_carve(lds_raw, n, P);
-    phi_update(d, f, job, iter + 1, true, L);
+    const PhiLds L = phi_carve(lds_raw, n, P, d.G);
+    phi_update(d, f, job, iter + 1, L);
 }
 
 /* ------------------------------------------------------- plain FP64 path */
@@ -816,7 +951,7 @@ __global__ __launch_bounds__(kBlock) void plain_subject_kernel(Dev d, uint32_t i
     const double tll = block_sum(acc, red);
     if (threadIdx.x == 0) {
         const int dst = info[1];
-        atomicAdd(d.evals, (unsigned long long)(t_hi - t_lo));
+        atomicAdd(d.evals + s, (unsigned long long)(t_hi - t_lo));
         const double cur = d.lp[(size_t)s * n + dst] + d.ll[(size_t)s * n + dst];
         if (mh_accept(misc[1] + tll, cur, misc[0])) {
             for (int p = 0; p < P; ++p) th[dst * P + p] = prop[p];
@@ -967,60 +1102,51 @@ __global__ void store_kernel(Dev d, uint32_t iter) {
 }
 
 /* Fixed statistics of hyper-only data rows [S x P] for population state f
- * (the data of run_hyper do not change, so they are reduced once). */
+ * (the data of run_hyper do not change, so they are reduced once): one
+ * thread per parameter, rows in order, two passes. */
 __global__ void hyper_data_stats_kernel(Dev d, const double *rows, int S, int f) {
-    __shared__ double red[kWaves];
-    __shared__ double mnv[kBlock], mxv[kBlock];
     const int P = d.P;
-    for (int p = 0; p < P; ++p) {
-        double sum = 0.0, nanc = 0.0;
-        for (int s = threadIdx.x; s < S; s += blockDim.x) {
+    for (int p = threadIdx.x; p < P; p += blockDim.x) {
+        double sum = 0.0, mn = __builtin_inf(), mx = -__builtin_inf();
+        bool nan = false;
+        for (int s = 0; s < S; ++s) {
             const double x = rows[(size_t)s * P + p];
             sum += x;
-            nanc += x != x ? 1.0 : 0.0;
-        }
-        const double mean = block_sum(sum, red) / (double)S;
-        const double nanv = block_sum(nanc, red);
-        double m2 = 0.0, mn = __builtin_inf(), mx = -__builtin_inf();
-        for (int s = threadIdx.x; s < S; s += blockDim.x) {
-            const double x = rows[(size_t)s * P + p];
-            m2 += (x - mean) * (x - mean);
             mn = fmin(mn, x);
             mx = fmax(mx, x);
+            nan |= x != x;
         }
-        m2 = block_sum(m2, red);
-        mnv[threadIdx.x] = mn;
-        mxv[threadIdx.x] = mx;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double a = __builtin_inf(), b = -__builtin_inf();
-            for (int t = 0; t < (int)blockDim.x; ++t) {
-                a = fmin(a, mnv[t]);
-                b = fmax(b, mxv[t]);
-            }
-            double *o = d.stats + (size_t)f * d.nchain * P * 4 + p * 4;
-            o[0] = nanv > 0.0 ? __builtin_nan("") : mean;
-            o[1] = m2;
-            o[2] = a;
-            o[3] = b;
+        const double mean = sum / (double)S;
+        double m2 = 0.0;
+        for (int s = 0; s < S; ++s) {
+            const double dx = rows[(size_t)s * P + p] - mean;
+            m2 += dx * dx;
         }
-        __syncthreads();
+        double *o = d.stats + (size_t)f * d.nchain * P * 4 + p * 4;
+        o[0] = nan ? __builtin_nan("") : mean;
+        o[1] = m2;
+        o[2] = mn;
+        o[3] = mx;
     }
 }
 
 /* ----------------------------------------------------------- sumloglike */
-template <int MODE, int TPL>
-__global__ __launch_bounds__(kBlock) void loglike_kernel(Dev d, const double *thetas, int s,
+/* out[q] = log-likelihood of thetas[q] over subject s's trials, with the
+ * sampler's canonical chunk sums (bit-identical to the sampler's values). */
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void loglike_kernel(Dev d, const double *thetas, int s, int cmax,
                                                         double *out) {
     extern __shared__ __attribute__((aligned(16))) char lds_raw[];
     const int P = d.P;
-    const SweepLds L = carve(lds_raw, 1, P, 1, d.n_cond, d.n_acc, 1);
+    const SweepLds L = carve(lds_raw, 1, P, 1, d.n_cond * d.n_acc, 1, cmax);
     const int q = blockIdx.x;
     for (int p = threadIdx.x; p < P; p += blockDim.x) L.prop[p] = thetas[(size_t)q * P + p];
     if (threadIdx.x == 0) L.active[0] = 1;
     __syncthreads();
-    tile_loglike<MODE, TPL>(d, L, 1, 1, d.subj_off[s], d.subj_off[s + 1]);
-    if (threadIdx.x == 0) out[q] = L.tile[0];
+    const int64_t t0 = d.subj_off[s], t1 = d.subj_off[s + 1];
+    const int nj = (int)((t1 - t0 + kChunk - 1) / kChunk);
+    proposals_loglike<MODE>(d, L, 1, 1, cmax, t0, t1, 0, nj, false, 0, 0);
+    if (threadIdx.x == 0) out[q] = L.tll[0];
 }
 
 } // namespace ggdmc
@@ -1030,27 +1156,24 @@ using namespace ggdmc;
 
 extern "C" {
 
-size_t ggdmc_internal_sweep_lds(int n, int P, int ng, int n_cond, int n_acc, int chain_batch) {
-    return sweep_lds_bytes(n, P, ng, n_cond, n_acc, chain_batch);
+size_t ggdmc_internal_sweep_lds(int n, int P, int kmax, int per_chain, int chain_batch, int cmax) {
+    return sweep_lds_bytes(n, P, kmax, per_chain, chain_batch, cmax);
 }
-size_t ggdmc_internal_phi_lds(int n, int P) { return phi_lds_bytes(n, P); }
+size_t ggdmc_internal_phi_lds(int n, int P, int G) { return phi_lds_bytes(n, P, G); }
 
-hipError_t ggdmc_internal_launch_sweep(const Dev *d, const SweepArgs *a, int mode, int tpl,
-                                       size_t lds, hipStream_t stream) {
-    dim3 grid(d->n_tiles), block(kBlock);
-    if (mode == GGDMC_MIXED_BATCHED) {
-        if (tpl == 2)
-            subject_sweep_kernel<MODE_MIXED, 2><<<grid, block, lds, stream>>>(*d, *a);
-        else
-            subject_sweep_kernel<MODE_MIXED, 1><<<grid, block, lds, stream>>>(*d, *a);
-    } else {
-        subject_sweep_kernel<MODE_NAIVE, 1><<<grid, block, lds, stream>>>(*d, *a);
-    }
+hipError_t ggdmc_internal_launch_sweep(const Dev *d, const SweepArgs *a, int mode, size_t lds,
+                                       hipStream_t stream) {
+    const dim3 grid(a->n_items), block(kBlock);
+    if (mode == GGDMC_MIXED_BATCHED)
+        subject_sweep_kernel<MODE_MIXED><<<grid, block, lds, stream>>>(*d, *a);
+    else
+        subject_sweep_kernel<MODE_NAIVE><<<grid, block, lds, stream>>>(*d, *a);
     return hipGetLastError();
 }
 
 hipError_t ggdmc_internal_launch_phi(const Dev *d, uint32_t iter_off, size_t lds, hipStream_t stream) {
-    phi_kernel<<<dim3(d->n_phi * d->nchain), dim3(kBlock), lds, stream>>>(*d, iter_off);
+    const int nblk = (d->nchain * d->P + kBlock - 1) / kBlock;
+    phi_kernel<<<dim3(d->n_phi * nblk), dim3(kBlock), lds, stream>>>(*d, iter_off, nblk);
     return hipGetLastError();
 }
 
@@ -1079,11 +1202,12 @@ hipError_t ggdmc_internal_launch_hyper_stats(const Dev *d, const double *rows, i
 }
 
 hipError_t ggdmc_internal_launch_loglike(const Dev *d, const double *thetas, int nq, int s,
-                                         int mode, size_t lds, hipStream_t stream, double *out) {
+                                         int mode, int cmax, size_t lds, hipStream_t stream,
+                                         double *out) {
     if (mode == GGDMC_MIXED_BATCHED)
-        loglike_kernel<MODE_MIXED, 2><<<dim3(nq), dim3(kBlock), lds, stream>>>(*d, thetas, s, out);
+        loglike_kernel<MODE_MIXED><<<dim3(nq), dim3(kBlock), lds, stream>>>(*d, thetas, s, cmax, out);
     else
-        loglike_kernel<MODE_NAIVE, 1><<<dim3(nq), dim3(kBlock), lds, stream>>>(*d, thetas, s, out);
+        loglike_kernel<MODE_NAIVE><<<dim3(nq), dim3(kBlock), lds, stream>>>(*d, thetas, s, cmax, out);
     return hipGetLastError();
 }
 

@@ -147,11 +147,13 @@ class _Out:
 
 def make_config(nmc, nchain, thin=1, pop_migration_prob=0.0, sub_migration_prob=0.0,
                 gamma_precursor=2.38, rp=0.001, is_hblocked=False, is_pblocked=False,
-                precision=_lib.MIXED_BATCHED, sweep_groups=0, trials_per_tile=0, iter_offset=0):
+                precision=_lib.MIXED_BATCHED, sweep_groups=0, trials_per_tile=0,
+                chains_per_item=0, iter_offset=0):
     return _lib.Config(int(nmc), int(thin), int(nchain), float(pop_migration_prob),
                        float(sub_migration_prob), float(gamma_precursor), float(rp),
                        int(bool(is_hblocked)), int(bool(is_pblocked)), int(precision),
-                       int(sweep_groups), int(trials_per_tile), int(iter_offset))
+                       int(sweep_groups), int(trials_per_tile), int(chains_per_item),
+                       int(iter_offset))
 
 
 @dataclass
@@ -249,11 +251,14 @@ class Session:
         check(_lib.lib().ggdmc_session_trial_evals(self.h, C.byref(v)))
         return v.value
 
-    def tile(self):
-        a = C.c_int32()
-        b = C.c_int32()
-        check(_lib.lib().ggdmc_session_tile(self.h, C.byref(a), C.byref(b)))
-        return a.value, b.value
+    def decomposition(self):
+        """(chains per item or -1 for one item per subject, trials per item or 0
+        for whole subjects, items of the largest launch)."""
+        k = C.c_int32()
+        t = C.c_int32()
+        m = C.c_int32()
+        check(_lib.lib().ggdmc_session_decomposition(self.h, C.byref(k), C.byref(t), C.byref(m)))
+        return k.value, t.value, m.value
 
     def download(self):
         """Returns per job {'phi': Posterior|None, 'subjects': [Posterior]}."""

@@ -60,7 +60,11 @@ typedef struct {
     int32_t sweep_groups;   /* chain groups per sweep: 0 = default (nchain for
                                FP64_PLAIN, 2 otherwise); nchain = the reference's
                                sequential in-place order */
-    int32_t trials_per_tile; /* 0 = autotuned */
+    int32_t trials_per_tile; /* trials per work item: 0 = whole subject (autotuned when
+                                chains_per_item is 0 too); rounded up to a multiple of 128 */
+    int32_t chains_per_item; /* chains of a chain group per work item: 0 = all (autotuned
+                                when trials_per_tile is 0 too); -1 = one item per subject
+                                covering every group in one launch */
     uint64_t iter_offset;   /* Philox iteration offset (restart continues a stream) */
 } ggdmc_config;
 
@@ -161,8 +165,13 @@ int ggdmc_session_profile(ggdmc_session *s, int32_t n_iter, double *subject_kern
 int ggdmc_session_trial_evals(ggdmc_session *s, double *evals);
 void ggdmc_session_destroy(ggdmc_session *s);
 
-/* Autotuned trials per tile for a session (after create). */
-int ggdmc_session_tile(ggdmc_session *s, int32_t *trials_per_tile, int32_t *n_tiles);
+/* Work decomposition of a session's subject launches (after create, so after
+ * autotuning): chains per item (-1: one item per subject over every group),
+ * trials per item (0: whole subject) and the items of the largest launch.
+ * The decomposition never changes results: log-likelihoods are summed in a
+ * fixed chunk order whatever the split. */
+int ggdmc_session_decomposition(ggdmc_session *s, int32_t *chains_per_item,
+                                int32_t *trials_per_tile, int32_t *items_per_launch);
 
 const char *ggdmc_last_error(void);
 /* Number of visible gfx950 devices (0 without a GPU; never fails). */

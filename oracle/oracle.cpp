@@ -80,7 +80,11 @@ inline double u32_open(uint32_t w) { return ((double)w + 0.5) * 2.32830643653869
 inline uint32_t below(uint32_t w, uint32_t n) { return (uint32_t)(((uint64_t)w * n) >> 32); }
 
 /* ------------------------------------------------------- normal helpers */
-inline double pnorm_(double z) { return 0.5 * std::erfc(-z * M_SQRT1_2); }
+/* Phi as R's pnorm (nmath/pnorm.c) rounds it: the lower tail directly, above
+ * 0 as 1 - the accurate upper tail. */
+inline double pnorm_(double z) {
+    return z > 0.0 ? 1.0 - 0.5 * std::erfc(z * M_SQRT1_2) : 0.5 * std::erfc(-z * M_SQRT1_2);
+}
 inline double dnorm_(double z) { return kInvSqrt2Pi * std::exp(-0.5 * z * z); }
 
 /* ------------------------------------------------------ LBA likelihood */

@@ -13,12 +13,31 @@ struct HostOps {
     static float log2(float x) { return std::log2(x); }
     static float rcp(float x) { return 1.0f / x; }
     static double erfc(double x) { return std::erfc(x); }
-    static double pnorm(double z) { return 0.5 * std::erfc(-z * ggdmc::kSqrt1_2); }
+    static double pnorm(double z) {
+        return z > 0.0 ? 1.0 - 0.5 * std::erfc(z * ggdmc::kSqrt1_2) : 0.5 * std::erfc(-z * ggdmc::kSqrt1_2);
+    }
     static double dnorm(double z) { return ggdmc::kInvSqrt2Pi * std::exp(-0.5 * z * z); }
 };
 } // namespace
 
 extern "C" {
+
+// One accumulator at one response time: the mixed-precision winner log density
+// and loser survivor, and the naive FP64 (reference formula) pdf and CDF.
+void host_acc_eval(const double *pars, double rt, double *out4) { /* out4: 6 values */
+    using namespace ggdmc;
+    const double theta[6] = {pars[0], pars[1], pars[2], pars[3], 0.0, pars[4]};
+    const int32_t src[6] = {0, 1, 2, 3, 4, 5};
+    const double cst[6] = {0, 0, 0, 0, 0, 0};
+    AccK k;
+    make_acck<HostOps>(theta, src, cst, k);
+    bool ill_w = false, ill_l = false;
+    out4[0] = mixed_winner_logpdf<HostOps>(k, rt, ill_w);
+    out4[1] = mixed_loser_survivor<HostOps>(k, rt, ill_l);
+    naive_pdf_cdf<HostOps>(k, rt - k.t0, out4[2], out4[3]);
+    out4[4] = ill_w;
+    out4[5] = ill_l;
+}
 
 // Per-trial log terms (mixed precision) for one subject; returns the sum.
 double host_mixed_terms(const int32_t *winner, const int32_t *par_src, const double *par_const,
@@ -33,25 +52,36 @@ double host_mixed_terms(const int32_t *winner, const int32_t *par_src, const dou
                                par_const + (c * n_acc + j) * 6, ks[j]);
         const int w = winner[c];
         for (int64_t i = cell_off[c]; i < cell_off[c + 1]; ++i) {
-            double lp = mixed_winner_logpdf<HostOps>(ks[w], rt[i]);
-            double acc = 0.0;
-            int neg = 0;
-            bool zero = false, nan = lp != lp;
-            for (int j = 0; j < n_acc; ++j) {
-                if (j == w) continue;
-                const double S = mixed_loser_survivor<HostOps>(ks[j], rt[i]);
-                if (S != S) nan = true;
-                const double aS = std::fabs(S);
-                fold_survivor(S, aS > 0.0 ? mixed_log<HostOps>(aS) : 0.0, acc, neg, zero);
-            }
-            double term;
-            if (nan) term = NAN;
-            else if (zero || neg) term = kLogDblEps;
-            else term = lp + acc;
+            const double term = mixed_trial_term<HostOps>(ks.data(), n_acc, w, rt[i]);
             if (out) out[i] = term;
             sum += term;
         }
     }
     return sum;
+}
+
+// Per-trial flags: 1 where the batched path evaluates the trial with the
+// reference formula (an ill-conditioned accumulator); returns their count.
+int64_t host_mixed_flags(const int32_t *winner, const int32_t *par_src, const double *par_const,
+                         int32_t n_cell, int32_t n_acc, const double *rt, const int64_t *cell_off,
+                         const double *theta, int32_t *flags) {
+    using namespace ggdmc;
+    std::vector<AccK> ks(n_acc);
+    int64_t n = 0;
+    for (int c = 0; c < n_cell; ++c) {
+        for (int j = 0; j < n_acc; ++j)
+            make_acck<HostOps>(theta, par_src + (c * n_acc + j) * 6,
+                               par_const + (c * n_acc + j) * 6, ks[j]);
+        const int w = winner[c];
+        for (int64_t i = cell_off[c]; i < cell_off[c + 1]; ++i) {
+            bool ill = false;
+            (void)mixed_winner_logpdf<HostOps>(ks[w], rt[i], ill);
+            for (int j = 0; j < n_acc; ++j)
+                if (j != w) (void)mixed_loser_survivor<HostOps>(ks[j], rt[i], ill);
+            if (flags) flags[i] = ill;
+            n += ill;
+        }
+    }
+    return n;
 }
 }

@@ -163,3 +163,45 @@ def test_mixed_hier_stored_ll_within_tolerance(gpu):
     print("mixed hierarchical: stored-LL max rel %.2e" % worst)
     assert worst <= 1e-6
     assert moved > 0
+
+
+@pytest.mark.parametrize("precision", ["MIXED_BATCHED", "FP64_BATCHED"])
+def test_decomposition_never_changes_results(gpu, precision):
+    # work items (chain ranges x trial tiles, or one item per subject over every
+    # group) only change the schedule: log-likelihoods are canonical chunk sums
+    fix = fx.load("lba_data6")
+    S, n = 5, 12
+    (phi, plp, pll), subs = fx.hier_start(fix, S, n, seed=21)
+    datas = [fix.pop_data(s) for s in range(S)]
+    prec = getattr(gpu, precision)
+    ref = None
+    for k, tpt in ((-1, 0), (0, 0), (1, 0), (4, 128), (2, 256), (0, 384)):
+        cfg = gpu.make_config(nmc=6, nchain=n, pop_migration_prob=0.2, sub_migration_prob=0.3,
+                              precision=prec, sweep_groups=2, chains_per_item=k,
+                              trials_per_tile=tpt)
+        r = gpu.run(cfg, fix.model, fix.pop_prior, fix.hyper_prior, datas, (phi, plp, pll), subs,
+                    seed=31)
+        if ref is None:
+            ref = r
+            continue
+        assert np.array_equal(r["phi"].theta, ref["phi"].theta), (k, tpt)
+        assert np.array_equal(r["phi"].log_likelihoods, ref["phi"].log_likelihoods), (k, tpt)
+        for a, b in zip(r["subject_theta"], ref["subject_theta"]):
+            assert np.array_equal(a.theta, b.theta), (k, tpt)
+            assert np.array_equal(a.log_likelihoods, b.log_likelihoods), (k, tpt)
+            assert np.array_equal(a.summed_log_prior, b.summed_log_prior), (k, tpt)
+
+
+def test_loglike_api_equals_sampler_values(gpu):
+    # ggdmc_loglike uses the sampler's chunk sums: a stored state's LL is
+    # reproduced bit for bit
+    fix = fx.load("lba_data6")
+    n = 12
+    start = fx.near_start(fix, n, seed=3)
+    cfg = _cfg(gpu, nchain=n, precision=gpu.MIXED_BATCHED, sweep_groups=2, nmc=8)
+    post = gpu.run_subject(cfg, fix.model, fix.sub_prior, fix.sub_data, start, seed=5)
+    th = post.theta[:, :, -1].T
+    ll = gpu.loglike(fix.model, fix.sub_data, th, gpu.MIXED_BATCHED)
+    moved = post.log_likelihoods[:, -1] != start[2]
+    assert moved.any()
+    assert np.array_equal(ll[moved], post.log_likelihoods[moved, -1])

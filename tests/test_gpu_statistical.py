@@ -24,61 +24,80 @@ def _moments_agree(a, b, burn):
     return z, r
 
 
+def _last(post):
+    return post.theta[:, :, -1], post.summed_log_prior[:, -1], post.log_likelihoods[:, -1]
+
+
 def test_single_subject_posterior_matches_oracle(gpu):
+    # The reference's workflow (tests/testthat/Group5/*.r): a burn-in with
+    # migration, then sampling without it. Migration is not a reversible move,
+    # so its stationary bias depends on the sweep order; the migration-free
+    # sampling phase of the two-group batched sampler (mixed precision) and of
+    # the oracle in the reference's sequential order share the posterior.
     w = workloads.make("c2", ll_fn=_oracle_ll)
     job = w.jobs[0]
-    nmc = 1500
-    cfg = w.config(nmc=nmc, precision=gpu.MIXED_BATCHED, sweep_groups=2)
-    runs = gpu.run_batch(cfg, w.model, [gpu.JobSpec(job.kind, 100 + k, job.p_prior, start=job.start,
-                                                    subject=job.subject, nparameter_de=13)
-                                        for k in range(3)])
+    n1, n2 = 1000, 3000
+    cfg1 = gpu.make_config(nmc=n1, nchain=w.nchain, sub_migration_prob=0.06,
+                           precision=gpu.MIXED_BATCHED, sweep_groups=2)
+    cfg2 = gpu.make_config(nmc=n2, nchain=w.nchain, precision=gpu.MIXED_BATCHED, sweep_groups=2,
+                           iter_offset=n1)
+    specs = [gpu.JobSpec(job.kind, 100 + k, job.p_prior, start=job.start, subject=job.subject,
+                         nparameter_de=13) for k in range(3)]
+    burn = gpu.run_batch(cfg1, w.model, specs)
+    specs2 = [gpu.JobSpec(job.kind, 100 + k, job.p_prior, start=_last(b["subjects"][0]),
+                          subject=job.subject, nparameter_de=13) for k, b in enumerate(burn)]
+    runs = gpu.run_batch(cfg2, w.model, specs2)
     g_theta = np.concatenate([r["subjects"][0].theta for r in runs], axis=1)
-    ocfg = O.config(nmc=nmc, nchain=w.nchain, nparameter_de=13, sub_migration_prob=0.06,
-                    sweep_groups=w.nchain, seed=7)
     om = O.Model(w.model.winner, w.model.par_src, w.model.par_const, 13)
-    o_theta, _, _ = O.run_subject(ocfg, om, fx.oprior(job.p_prior), job.subject.rt,
-                                  job.subject.cell_off, *job.start)
-    burn = 500
-    rhat = diagnostics.psrf(g_theta, burn)
-    z, r = _moments_agree(g_theta, o_theta, burn)
+    o1 = O.run_subject(O.config(nmc=n1, nchain=w.nchain, nparameter_de=13, sub_migration_prob=0.06,
+                                sweep_groups=w.nchain, seed=7),
+                       om, fx.oprior(job.p_prior), job.subject.rt, job.subject.cell_off, *job.start)
+    o_theta, _, _ = O.run_subject(O.config(nmc=n2, nchain=w.nchain, nparameter_de=13,
+                                           sweep_groups=w.nchain, seed=7, iter_offset=n1),
+                                  om, fx.oprior(job.p_prior), job.subject.rt, job.subject.cell_off,
+                                  o1[0][:, :, -1], o1[1][:, -1], o1[2][:, -1])
+    rhat = diagnostics.psrf(g_theta, 0)
+    z, r = _moments_agree(g_theta, o_theta, 0)
     print("R-hat max %.3f; mean z max %.2f; sd ratio %.3f..%.3f" % (rhat.max(), z.max(), r.min(), r.max()))
     assert rhat.max() < 1.1
     assert z.max() < 5.0
     assert np.all(np.abs(r - 1) < 0.15)
 
 
-def test_hierarchical_posterior_matches_oracle(gpu):
-    w = workloads.make("c3", scale=0.25, ll_fn=_oracle_ll)     # 8 subjects x 768 trials
+def test_hierarchical_posterior_matches_reference_order(gpu):
+    # 32 subjects x 768 trials: the two-group mixed-precision sampler against
+    # the FP64 batched sampler in the reference's sequential order (whose
+    # trajectory follows the oracle draw for draw, test_gpu_sampler.py), three
+    # independent runs each, no migration
+    w = workloads.make("c3")
     job = w.jobs[0]
-    nmc = 800
-    cfg = w.config(nmc=nmc, precision=gpu.MIXED_BATCHED, sweep_groups=2)
-    r = gpu.run_batch(cfg, w.model, [job])[0]
-    ocfg = O.config(nmc=nmc, nchain=w.nchain, nparameter_de=26, sweep_groups=w.nchain, seed=5)
-    om = O.Model(w.model.winner, w.model.par_src, w.model.par_const, 13)
-    datas = job.subjects
-    rt = np.concatenate([d.rt for d in datas])
-    so = np.concatenate([[0], np.cumsum([len(d.rt) for d in datas])])
-    co = np.stack([d.cell_off for d in datas])
-    o = O.run_hier(ocfg, om, fx.oprior(job.p_prior), fx.oprior(job.h_prior), rt, so, co,
-                   *job.phi_start, np.stack([s[0] for s in job.subject_starts]),
-                   np.stack([s[1] for s in job.subject_starts]),
-                   np.stack([s[2] for s in job.subject_starts]))
-    burn = 300
-    # population locations and one subject's parameters
-    zl, rl = _moments_agree(r["phi"].theta[:13], o[0][:13], burn)
-    zs, rs = _moments_agree(r["subject_theta"][0].theta, o[3][0], burn)
-    print("phi loc z max %.2f, sd ratio %.2f..%.2f; subject z max %.2f"
-          % (zl.max(), rl.min(), rl.max(), zs.max()))
-    assert diagnostics.psrf(r["phi"].theta[:13], burn).max() < 1.1
+    nmc, burn = 4000, 2000
+    out = {}
+    for name, prec, G in (("mixed", gpu.MIXED_BATCHED, 2), ("fp64_seq", gpu.FP64_BATCHED, w.nchain)):
+        cfg = gpu.make_config(nmc=nmc, nchain=w.nchain, precision=prec, sweep_groups=G)
+        specs = [gpu.JobSpec(job.kind, 500 + 10 * k + G, job.p_prior, h_prior=job.h_prior,
+                             subjects=job.subjects, phi_start=job.phi_start,
+                             subject_starts=job.subject_starts, nparameter_de=26)
+                 for k in range(3)]
+        rs = gpu.run_batch(cfg, w.model, specs)
+        out[name] = (np.concatenate([r["phi"].theta for r in rs], axis=1),
+                     np.concatenate([r["subjects"][0].theta for r in rs], axis=1))
+    for name in out:
+        print(name, "phi R-hat max %.3f" % diagnostics.psrf(out[name][0], burn).max())
+    zl, rl = _moments_agree(out["mixed"][0][:13], out["fp64_seq"][0][:13], burn)
+    zs, rs_ = _moments_agree(out["mixed"][1], out["fp64_seq"][1], burn)
+    print("phi loc z max %.2f, sd ratio %.2f..%.2f; subject z max %.2f, sd ratio %.2f..%.2f"
+          % (zl.max(), rl.min(), rl.max(), zs.max(), rs_.min(), rs_.max()))
+    assert diagnostics.psrf(out["mixed"][0], burn).max() < 1.1
     assert zl.max() < 5.0 and zs.max() < 5.0
-    assert np.all(np.abs(rl - 1) < 0.25) and np.all(np.abs(rs - 1) < 0.25)
+    assert np.all(np.abs(rl - 1) < 0.25) and np.all(np.abs(rs_ - 1) < 0.25)
 
 
 def test_batched_job_equals_job_alone(gpu):
     # a run's trajectory does not depend on the other runs of its batch
     w = workloads.make("c4", scale=1 / 16, ll_fn=_oracle_ll)    # 16 ragged subjects + 4 fits
-    cfg = w.config(nmc=6, precision=gpu.MIXED_BATCHED, sweep_groups=2, trials_per_tile=256,
-                   sub_migration_prob=0.2, pop_migration_prob=0.2)
+    cfg = gpu.make_config(nmc=6, nchain=w.nchain, precision=gpu.MIXED_BATCHED, sweep_groups=2,
+                          trials_per_tile=256, sub_migration_prob=0.2, pop_migration_prob=0.2)
     together = gpu.run_batch(cfg, w.model, w.jobs)
     for k in (0, len(w.jobs) - 1):
         alone = gpu.run_batch(cfg, w.model, [w.jobs[k]])[0]
