@@ -11,6 +11,7 @@
  *   lp, ll  double [S][n]
  *   phi     double [F][n][2P]   population states of hierarchical / hyper runs
  *   stats   double [F][n][P][4] per-chain sufficient statistics of the subject states
+ *   bstats  double [F][n*P][blocks][4] the same per block of 32 subjects
  *   csum    double [chunks][n]  chunk sums of trial-split work items
  *   out_*   [S][nmc][n][P] etc. stored samples, R's column-major order per subject
  *
@@ -32,6 +33,7 @@ constexpr int kBlock = 256;        /* threads per workgroup of the sweep kernels
 constexpr int kWaves = kBlock / 64;
 constexpr int kMaxChains = 256;    /* chains per run */
 constexpr int kChunk = 128;        /* trials per canonical chunk sum */
+constexpr int kStatBlock = 32;     /* subjects per block of the population statistics */
 
 struct JobDev {
     int32_t kind;           /* GGDMC_KIND_* */
@@ -71,6 +73,8 @@ struct Dev {
     double *theta, *lp, *ll;
     double *phi, *phi_lp, *phi_ll;
     double *stats;              /* [F][n][P][4]: mean, M2, min, max over subjects */
+    double *bstats;             /* [F][bstats_stride]: per (pair, subject block) statistics */
+    int64_t bstats_stride;      /* doubles per run: n * P * blocks of the largest run * 4 */
     double *csum;
 
     uint32_t *subj_ctr;         /* [S] items of a subject finished (final launch) */

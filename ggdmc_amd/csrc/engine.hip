@@ -404,6 +404,13 @@ void ggdmc_session::build(const ggdmc_config *c, const ggdmc_model *model, const
     d.phi_ll = pool.upload(pll0, stream);
     d.stats = pool.alloc<double>((size_t)std::max(F, 1) * n * P * 4);
     {
+        int max_sub = 1;
+        for (const auto &j : jobs)
+            if (j.kind == GGDMC_KIND_HIER) max_sub = std::max(max_sub, (int)j.nsubject);
+        d.bstats_stride = (int64_t)n * P * ((max_sub + kStatBlock - 1) / kStatBlock) * 4;
+        d.bstats = pool.alloc<double>((size_t)std::max(F, 1) * d.bstats_stride);
+    }
+    {
         uint32_t *ctr = pool.alloc<uint32_t>((size_t)S + F + nj);
         HIPCHECK(hipMemsetAsync(ctr, 0, sizeof(uint32_t) * ((size_t)S + F + nj), stream));
         d.subj_ctr = ctr;

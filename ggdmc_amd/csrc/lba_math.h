@@ -275,11 +275,11 @@ GGDMC_FN double naive_trial_term(const AccK *kc, int n_acc, int w, double rt) {
     return std::log(dens <= 0.0 ? 2.220446049250313e-16 : dens);
 }
 
-/* Batched-path trial term: mixed evaluation, or the reference formula when
- * any accumulator's evaluation is ill-conditioned. */
+/* Batched-path trial term, mixed evaluation; ill is set when any
+ * accumulator's evaluation is ill-conditioned (the caller then takes
+ * naive_trial_term instead). NaN propagates without the flag. */
 template <class Ops>
-GGDMC_FN double mixed_trial_term(const AccK *kc, int n_acc, int w, double rt) {
-    bool ill = false;
+GGDMC_FN double mixed_trial_term_flagged(const AccK *kc, int n_acc, int w, double rt, bool &ill) {
     const double lp = mixed_winner_logpdf<Ops>(kc[w], rt, ill);
     double acc = 0.0, nanv = lp;
     int neg = 0;
@@ -294,9 +294,20 @@ GGDMC_FN double mixed_trial_term(const AccK *kc, int n_acc, int w, double rt) {
         const double aS = std::fabs(S);
         fold_survivor(S, aS > 0.0 ? mixed_log<Ops>(aS) : 0.0, acc, neg, zero);
     }
-    if (nan) return nanv;
-    if (ill) return naive_trial_term<Ops>(kc, n_acc, w, rt);
+    if (nan) {
+        ill = false;
+        return nanv;
+    }
     return (zero || neg) ? kLogDblEps : lp + acc;
+}
+
+/* Batched-path trial term: mixed evaluation, or the reference formula when
+ * any accumulator's evaluation is ill-conditioned. */
+template <class Ops>
+GGDMC_FN double mixed_trial_term(const AccK *kc, int n_acc, int w, double rt) {
+    bool ill = false;
+    const double v = mixed_trial_term_flagged<Ops>(kc, n_acc, w, rt, ill);
+    return ill ? naive_trial_term<Ops>(kc, n_acc, w, rt) : v;
 }
 
 

@@ -79,7 +79,9 @@ template <int MODE>
 __device__ void chunk_sums(const Dev &d, int64_t t0, int64_t t_end, int j_lo, int nj, int kb,
                            const int *active, const double *acck, double *C, int c_stride) {
     static_assert(kChunk == 128, "two trials per lane");
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int lane = threadIdx.x & 63;
+    /* the wave index is uniform: keep the unit loop scalar */
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nw = blockDim.x >> 6;
     const int per_chain = d.n_cond * d.n_acc;
     for (int u = wave; u < kb * nj; u += nw) {
         const int c = u / nj, jj = u - c * nj;
@@ -88,13 +90,20 @@ __device__ void chunk_sums(const Dev &d, int64_t t0, int64_t t_end, int j_lo, in
             const AccK *kc = reinterpret_cast<const AccK *>(acck) + (size_t)c * per_chain;
             const int64_t i1 = t0 + (int64_t)(j_lo + jj) * kChunk + lane, i2 = i1 + 64;
             double a = 0.0, b = 0.0;
-            if (i1 < t_end) {
-                const int code = d.tcode[i1];
-                a = trial_term<MODE>(kc + (code >> 3) * d.n_acc, d.n_acc, code & 7, d.rt[i1]);
-            }
-            if (i2 < t_end) {
-                const int code = d.tcode[i2];
-                b = trial_term<MODE>(kc + (code >> 3) * d.n_acc, d.n_acc, code & 7, d.rt[i2]);
+            const bool h1 = i1 < t_end, h2 = i2 < t_end;
+            const int code1 = h1 ? d.tcode[i1] : 0, code2 = h2 ? d.tcode[i2] : 0;
+            const double rt1 = h1 ? d.rt[i1] : 0.0, rt2 = h2 ? d.rt[i2] : 0.0;
+            const AccK *k1 = kc + (code1 >> 3) * d.n_acc, *k2 = kc + (code2 >> 3) * d.n_acc;
+            if (MODE == MODE_MIXED) {
+                bool ill1 = false, ill2 = false;
+                if (h1) a = mixed_trial_term_flagged<DevOps>(k1, d.n_acc, code1 & 7, rt1, ill1);
+                if (h2) b = mixed_trial_term_flagged<DevOps>(k2, d.n_acc, code2 & 7, rt2, ill2);
+                /* rare: ill-conditioned trials take the reference formula */
+                if (ill1) a = naive_trial_term<DevOps>(k1, d.n_acc, code1 & 7, rt1);
+                if (ill2) b = naive_trial_term<DevOps>(k2, d.n_acc, code2 & 7, rt2);
+            } else {
+                if (h1) a = naive_trial_term<DevOps>(k1, d.n_acc, code1 & 7, rt1);
+                if (h2) b = naive_trial_term<DevOps>(k2, d.n_acc, code2 & 7, rt2);
             }
             v = a + b;
         }
@@ -104,9 +113,14 @@ __device__ void chunk_sums(const Dev &d, int64_t t0, int64_t t_end, int j_lo, in
 }
 
 /* ------------------------------------------------------------------ priors */
+/* Phi at +-inf without evaluating erfc (the same values: 1 and 0). */
+__device__ __forceinline__ double pnorm_bound(double z) {
+    return z == __builtin_inf() ? 1.0 : (z == -__builtin_inf() ? 0.0 : DevOps::pnorm(z));
+}
+
 /* Phi(b) - Phi(a) for a <= b; as Q(a) - Q(b) when a > 0 (no cancellation). */
 __device__ __forceinline__ double tnorm_mass(double a, double b) {
-    return a > 0.0 ? DevOps::pnorm(-a) - DevOps::pnorm(-b) : DevOps::pnorm(b) - DevOps::pnorm(a);
+    return a > 0.0 ? pnorm_bound(-a) - pnorm_bound(-b) : pnorm_bound(b) - pnorm_bound(a);
 }
 
 __device__ double log_prior1(int dist, double p0, double p1, double lo, double up, double x) {
@@ -578,33 +592,75 @@ __device__ double hyper_logprior(const Dev &d, const JobDev &job, const double *
     return s;
 }
 
-/* Sufficient statistics (mean, M2, min, max) of pairs q in [q0, q1)
- * (q = chain * P + parameter) over the run's S subjects: one thread per pair,
- * subjects in order, two passes. */
-__device__ void chain_stats_pairs(const Dev &d, const JobDev &job, int q0, int q1, double *out) {
+/* Sufficient statistics (mean, M2, min, max) of the run's S subject values
+ * of every (chain, parameter) pair q = chain * P + parameter, in a fixed
+ * order: subjects in blocks of kStatBlock (in subject order); a block's values
+ * are held in registers, summed in order, and its M2 taken about the block
+ * mean; blocks combine in order as
+ *   sum = sum_j sum_j,  mean = sum / S,  M2 = sum_j (M2_j + n_j (mean_j - mean)^2)
+ * (one block: the two-pass M2). Step 1: block statistics of items
+ * e = q * nblk + j in [e0, e1), one thread each, into bst[e][4]. */
+__device__ void block_stats(const Dev &d, const JobDev &job, int e0, int e1, double *bst) {
     const int P = d.P, n = d.nchain, S = job.nsubject;
+    const int nblk = (S + kStatBlock - 1) / kStatBlock;
     const size_t stride = (size_t)n * P;
-    for (int q = q0 + (int)threadIdx.x; q < q1; q += blockDim.x) {
-        const double *src = d.theta + (size_t)job.first_subject * stride + q;
+    for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x) {
+        const int q = e / nblk, j = e - q * nblk;
+        const int s0 = j * kStatBlock, nb = min(kStatBlock, S - s0);
+        const double *src = d.theta + (size_t)(job.first_subject + s0) * stride + q;
+        double v[kStatBlock];
+#pragma unroll
+        for (int i = 0; i < kStatBlock; ++i) v[i] = i < nb ? src[i * stride] : 0.0;
         double sum = 0.0, mn = __builtin_inf(), mx = -__builtin_inf();
         bool nan = false;
-#pragma unroll 8
-        for (int s = 0; s < S; ++s) {
-            const double x = src[s * stride];
-            sum += x;
-            mn = fmin(mn, x);
-            mx = fmax(mx, x);
-            nan |= x != x;
+#pragma unroll
+        for (int i = 0; i < kStatBlock; ++i) {
+            if (i < nb) {
+                sum += v[i];
+                mn = fmin(mn, v[i]);
+                mx = fmax(mx, v[i]);
+                nan |= v[i] != v[i];
+            }
+        }
+        const double mean = sum / (double)nb;
+        double m2 = 0.0;
+#pragma unroll
+        for (int i = 0; i < kStatBlock; ++i) {
+            if (i < nb) {
+                const double dx = v[i] - mean;
+                m2 += dx * dx;
+            }
+        }
+        double *o = bst + (size_t)e * 4;
+        o[0] = nan ? __builtin_nan("") : sum;
+        o[1] = m2;
+        o[2] = mn;
+        o[3] = mx;
+    }
+}
+
+/* Step 2: combine the blocks of pairs [q0, q1) in order into out[q][4] =
+ * (mean, M2, min, max). */
+__device__ void combine_stats(const JobDev &job, int q0, int q1, const double *bst, double *out) {
+    const int S = job.nsubject;
+    const int nblk = (S + kStatBlock - 1) / kStatBlock;
+    for (int q = q0 + (int)threadIdx.x; q < q1; q += blockDim.x) {
+        const double *b = bst + (size_t)q * nblk * 4;
+        double sum = 0.0, mn = __builtin_inf(), mx = -__builtin_inf();
+        for (int j = 0; j < nblk; ++j) {
+            sum += b[j * 4];
+            mn = fmin(mn, b[j * 4 + 2]);
+            mx = fmax(mx, b[j * 4 + 3]);
         }
         const double mean = sum / (double)S;
         double m2 = 0.0;
-#pragma unroll 8
-        for (int s = 0; s < S; ++s) {
-            const double dx = src[s * stride] - mean;
-            m2 += dx * dx;
+        for (int j = 0; j < nblk; ++j) {
+            const int nb = min(kStatBlock, S - j * kStatBlock);
+            const double dm = b[j * 4] / (double)nb - mean;
+            m2 += b[j * 4 + 1] + (double)nb * dm * dm;
         }
         double *o = out + (size_t)q * 4;
-        o[0] = nan ? __builtin_nan("") : mean;
+        o[0] = mean;
         o[1] = m2;
         o[2] = mn;
         o[3] = mx;
@@ -819,14 +875,15 @@ __device__ void phi_update(const Dev &d, int f, const JobDev &job, uint32_t iter
     }
 }
 
-/* grid = n_phi * nblk: block (f, b) computes the statistics of pairs
- * [b * blockDim, (b + 1) * blockDim) of a hierarchical run; the last block of
- * each run performs its population step. Hyper-only runs have fixed
- * statistics: their block b == 0 steps directly. */
-__global__ __launch_bounds__(kBlock) void phi_kernel(Dev d, uint32_t iter_off, int nblk) {
+/* grid = n_phi * nwg: workgroup (f, b) computes the block statistics of items
+ * [b * blockDim, (b + 1) * blockDim) of a hierarchical run; the last
+ * workgroup of each run combines them and performs the population step.
+ * Hyper-only runs have fixed statistics: their workgroup b == 0 steps
+ * directly. */
+__global__ __launch_bounds__(kBlock) void phi_kernel(Dev d, uint32_t iter_off, int nwg) {
     extern __shared__ __attribute__((aligned(16))) char lds_raw[];
     const int n = d.nchain, P = d.P;
-    const int f = blockIdx.x / nblk, b = blockIdx.x - f * nblk;
+    const int f = blockIdx.x / nwg, b = blockIdx.x - f * nwg;
     const JobDev job = d.jobs[d.phi_job[f]];
     const uint32_t iter = *d.iter_base + iter_off;
     const PhiLds L = phi_carve(lds_raw, n, P, d.G);
@@ -835,18 +892,22 @@ __global__ __launch_bounds__(kBlock) void phi_kernel(Dev d, uint32_t iter_off, i
         phi_update(d, f, job, iter, L);
         return;
     }
-    const int q0 = b * (int)blockDim.x, q1 = min(n * P, q0 + (int)blockDim.x);
-    chain_stats_pairs(d, job, q0, q1, d.stats + (size_t)f * n * P * 4);
+    const int nitem = n * P * ((job.nsubject + kStatBlock - 1) / kStatBlock);
+    double *bst = d.bstats + (size_t)f * d.bstats_stride;
+    if (b * (int)blockDim.x < nitem)
+        block_stats(d, job, b * (int)blockDim.x, min(nitem, (b + 1) * (int)blockDim.x), bst);
     __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) {
         const uint32_t old = atomicAdd(d.stats_ctr + f, 1u);
-        L.flag[0] = old == (uint32_t)(nblk - 1);
+        L.flag[0] = old == (uint32_t)(nwg - 1);
     }
     __syncthreads();
     if (!L.flag[0]) return;
     __threadfence();
     if (threadIdx.x == 0) d.stats_ctr[f] = 0u;
+    combine_stats(job, 0, n * P, bst, d.stats + (size_t)f * n * P * 4);
+    __syncthreads();
     phi_update(d, f, job, iter, L);
 }
 
@@ -871,7 +932,10 @@ __device__ void fused_phi_tail(const Dev &d, int x, const JobDev &job, uint32_t 
     if (threadIdx.x == 0) *ctr = 0u;
     if (iter + 1 > d.last_iter) return;
     const int f = job.phi_index, n = d.nchain, P = d.P;
-    chain_stats_pairs(d, job, 0, n * P, d.stats + (size_t)f * n * P * 4);
+    double *bst = d.bstats + (size_t)f * d.bstats_stride;
+    block_stats(d, job, 0, n * P * ((job.nsubject + kStatBlock - 1) / kStatBlock), bst);
+    __syncthreads();
+    combine_stats(job, 0, n * P, bst, d.stats + (size_t)f * n * P * 4);
     __syncthreads();
     const PhiLds L = phi_carve(lds_raw, n, P, d.G);
     phi_update(d, f, job, iter + 1, L);
@@ -1172,8 +1236,10 @@ hipError_t ggdmc_internal_launch_sweep(const Dev *d, const SweepArgs *a, int mod
 }
 
 hipError_t ggdmc_internal_launch_phi(const Dev *d, uint32_t iter_off, size_t lds, hipStream_t stream) {
-    const int nblk = (d->nchain * d->P + kBlock - 1) / kBlock;
-    phi_kernel<<<dim3(d->n_phi * nblk), dim3(kBlock), lds, stream>>>(*d, iter_off, nblk);
+    /* items of the largest run: pairs x subject blocks */
+    const int nwg = (int)((d->bstats_stride / 4 + kBlock - 1) / kBlock);
+    phi_kernel<<<dim3(d->n_phi * std::max(nwg, 1)), dim3(kBlock), lds, stream>>>(*d, iter_off,
+                                                                                std::max(nwg, 1));
     return hipGetLastError();
 }
 
