@@ -76,10 +76,10 @@ struct Dev {
     double *bstats;             /* [F][bstats_stride]: per (pair, subject block) statistics */
     int64_t bstats_stride;      /* doubles per run: n * P * blocks of the largest run * 4 */
     double *csum;
+    double *pcache;             /* [F][n][P][2]: log scale, log mass of each chain's subject prior */
+    const double *scache;       /* [prior entries][2]: the same for fixed priors (unif: log width) */
 
-    uint32_t *subj_ctr;         /* [S] items of a subject finished (final launch) */
-    uint32_t *stats_ctr;        /* [F] statistics blocks finished */
-    uint32_t *job_ctr;          /* [jobs] subjects of a run finished */
+    uint32_t *stats_ctr;        /* [F] statistics workgroups finished */
     uint32_t *tile_ctr;         /* [items] tiles of a (subject, chain range) finished */
     unsigned long long *evals;  /* [S] trial-likelihood evaluations per subject */
     const uint32_t *iter_base;  /* device iteration base (graph replays) */
@@ -100,10 +100,8 @@ struct SweepArgs {
     uint32_t iter_off;    /* iteration = *iter_base + iter_off */
     int32_t sweep;        /* blocked parameter index or 0 */
     int32_t g_begin, g_end;
-    int32_t final_launch; /* last subject launch of the iteration: store if due */
+    int32_t store_prev;   /* first launch of the iteration: store iteration - 1 if due */
     int32_t chain_batch;  /* chains whose accumulator constants fit in LDS at once */
-    int32_t fuse_phi;     /* final launch: the last subject of a hierarchical run
-                             performs the next iteration's population step */
     int32_t kmax;         /* most chains one item proposes */
     int32_t cmax;         /* chunk sums per chain held in LDS at once */
     int32_t n_items;

@@ -41,7 +41,8 @@ hipError_t ggdmc_internal_launch_plain_subject(const Dev *d, uint32_t iter, int 
 hipError_t ggdmc_internal_launch_plain_phi(const Dev *d, uint32_t iter, int sweep, int j,
                                            const double *rows, const int64_t *rows_off,
                                            hipStream_t stream);
-hipError_t ggdmc_internal_launch_store(const Dev *d, uint32_t iter, hipStream_t stream);
+hipError_t ggdmc_internal_launch_store(const Dev *d, uint32_t iter, int subjects_only,
+                                      hipStream_t stream);
 hipError_t ggdmc_internal_launch_hyper_stats(const Dev *d, const double *rows, int S, int f,
                                              hipStream_t stream);
 hipError_t ggdmc_internal_launch_loglike(const Dev *d, const double *thetas, int nq, int s,
@@ -122,8 +123,6 @@ struct ggdmc_session {
     size_t sweep_lds = 0, phi_lds = 0;
     int nsweep_sub = 1;
     int64_t total_chunks = 0;
-    bool fuse_phi = false;       /* population step fused into the final subject launch */
-    uint32_t phi_ready = 0;      /* iteration whose population step has been done */
     uint32_t next_iter = 1;      /* next local iteration to run (1-based) */
     uint32_t last_iter = 0;      /* nmc*thin - 1 */
     uint32_t *iter_base = nullptr;
@@ -152,7 +151,7 @@ struct ggdmc_session {
     void set_decomp(bool fused_new, int k_new, int tpt_new);
     void default_decomp();
     void enqueue_iteration(uint32_t off, std::vector<hipEvent_t> *ev = nullptr);
-    void ensure_phi_ready();
+    void flush_store();
     void run_batched(uint32_t n_iter);
     void run_plain(uint32_t n_iter);
     void autotune();
@@ -396,6 +395,25 @@ void ggdmc_session::build(const ggdmc_config *c, const ggdmc_model *model, const
     d.pr_p1 = pool.upload(pr_p1, stream);
     d.pr_lo = pool.upload(pr_lo, stream);
     d.pr_up = pool.upload(pr_up, stream);
+    {
+        /* cached logarithms of the fixed priors: unif log(upper - lower);
+         * tnorm log(scale), log(mass on [lower, upper]) with Phi as R's pnorm */
+        auto pnorm = [](double z) {
+            return z > 0.0 ? 1.0 - 0.5 * std::erfc(z * M_SQRT1_2) : 0.5 * std::erfc(-z * M_SQRT1_2);
+        };
+        std::vector<double> sc(2 * std::max<size_t>(pr_dist.size(), 1), 0.0);
+        for (size_t k = 0; k < pr_dist.size(); ++k) {
+            if (pr_dist[k] == GGDMC_DIST_UNIF) {
+                sc[2 * k] = std::log(pr_p1[k] - pr_p0[k]);
+            } else {
+                const double a = (pr_lo[k] - pr_p0[k]) / pr_p1[k], b = (pr_up[k] - pr_p0[k]) / pr_p1[k];
+                const double Z = a > 0.0 ? pnorm(-a) - pnorm(-b) : pnorm(b) - pnorm(a);
+                sc[2 * k] = std::log(pr_p1[k]);
+                sc[2 * k + 1] = std::log(Z);
+            }
+        }
+        d.scache = pool.upload(sc, stream);
+    }
     d.theta = pool.upload(th0, stream);
     d.lp = pool.upload(lp0, stream);
     d.ll = pool.upload(ll0, stream);
@@ -403,6 +421,7 @@ void ggdmc_session::build(const ggdmc_config *c, const ggdmc_model *model, const
     d.phi_lp = pool.upload(plp0, stream);
     d.phi_ll = pool.upload(pll0, stream);
     d.stats = pool.alloc<double>((size_t)std::max(F, 1) * n * P * 4);
+    d.pcache = pool.alloc<double>((size_t)std::max(F, 1) * n * P * 2);
     {
         int max_sub = 1;
         for (const auto &j : jobs)
@@ -410,13 +429,8 @@ void ggdmc_session::build(const ggdmc_config *c, const ggdmc_model *model, const
         d.bstats_stride = (int64_t)n * P * ((max_sub + kStatBlock - 1) / kStatBlock) * 4;
         d.bstats = pool.alloc<double>((size_t)std::max(F, 1) * d.bstats_stride);
     }
-    {
-        uint32_t *ctr = pool.alloc<uint32_t>((size_t)S + F + nj);
-        HIPCHECK(hipMemsetAsync(ctr, 0, sizeof(uint32_t) * ((size_t)S + F + nj), stream));
-        d.subj_ctr = ctr;
-        d.stats_ctr = ctr + S;
-        d.job_ctr = ctr + S + F;
-    }
+    d.stats_ctr = pool.alloc<uint32_t>((size_t)std::max(F, 1));
+    HIPCHECK(hipMemsetAsync(d.stats_ctr, 0, sizeof(uint32_t) * std::max(F, 1), stream));
     d.evals = pool.alloc<unsigned long long>((size_t)std::max(S, 1));
     HIPCHECK(hipMemsetAsync(d.evals, 0, sizeof(unsigned long long) * std::max(S, 1), stream));
     {
@@ -465,19 +479,6 @@ void ggdmc_session::build(const ggdmc_config *c, const ggdmc_model *model, const
     last_iter = (uint32_t)(cfg.nmc * cfg.thin - 1);
     d.last_iter = last_iter;
     next_iter = 1;
-    /* fuse the population step into the final subject launch when one
-     * workgroup reduces the statistics quickly (small runs), no hyper-only run
-     * needs the standalone population kernel, and the path is batched */
-    {
-        bool any_hyper = false;
-        int64_t work = 0;
-        for (const auto &j : jobs) {
-            any_hyper |= j.kind == GGDMC_KIND_HYPER;
-            if (j.kind == GGDMC_KIND_HIER) work = std::max<int64_t>(work, (int64_t)j.nsubject * n * P);
-        }
-        fuse_phi = mode != GGDMC_FP64_PLAIN && F > 0 && !any_hyper && work <= 65536;
-    }
-
     phi_lds = ggdmc_internal_phi_lds(n, P, G);
     require(F == 0 || phi_lds <= (size_t)device_lds_limit(), "population state too large for LDS");
     default_decomp();
@@ -558,24 +559,23 @@ void ggdmc_session::set_decomp(bool fused_new, int k_new, int tpt_new) {
     while (cmax > 1 && bytes() > soft) cmax = (cmax + 1) / 2;
     sweep_lds = bytes();
     require(sweep_lds <= limit, "design too large for LDS (chains x parameters)");
-    if (fuse_phi) sweep_lds = std::max(sweep_lds, phi_lds);
     drop_graph();
 }
 
-/* One iteration's launches. With fuse_phi the population step of this
- * iteration was done by the previous iteration's final launch (or by
- * ensure_phi_ready), and this iteration's final launch does the next one.
- * ev, if given, receives an event pair around every subject launch. */
+/* One iteration's launches: the population step of the iteration (from the
+ * subject states of the previous one), then one subject launch per (sweep,
+ * chain group), or per sweep with one item per subject over every group. The
+ * first subject launch stores the previous iteration's subject states when
+ * due. ev, if given, receives an event pair around every subject launch. */
 void ggdmc_session::enqueue_iteration(uint32_t off, std::vector<hipEvent_t> *ev) {
-    if (d.n_phi > 0 && !fuse_phi) HIPCHECK(ggdmc_internal_launch_phi(&d, off, phi_lds, stream));
+    if (d.n_phi > 0) HIPCHECK(ggdmc_internal_launch_phi(&d, off, phi_lds, stream));
     if (d.S == 0) return;
     const int lk = mode == GGDMC_FP64_BATCHED ? GGDMC_FP64_BATCHED : GGDMC_MIXED_BATCHED;
     const int ngl = fused ? 1 : d.G;
     for (int sw = 0; sw < nsweep_sub; ++sw) {
         for (int g = 0; g < ngl; ++g) {
-            const int fin = sw == nsweep_sub - 1 && g == ngl - 1;
-            SweepArgs a{off, sw, fused ? 0 : g, fused ? d.G : g + 1, fin, chain_batch,
-                        (int)(fin && fuse_phi), kmax, cmax, list_n[g], lists[g]};
+            SweepArgs a{off, sw, fused ? 0 : g, fused ? d.G : g + 1, (int)(sw == 0 && g == 0), chain_batch,
+                        kmax, cmax, list_n[g], lists[g]};
             hipEvent_t e0 = nullptr, e1 = nullptr;
             if (ev) {
                 HIPCHECK(hipEventCreate(&e0));
@@ -590,13 +590,12 @@ void ggdmc_session::enqueue_iteration(uint32_t off, std::vector<hipEvent_t> *ev)
     }
 }
 
-/* With fuse_phi: runs the population step of next_iter if no earlier launch did. */
-void ggdmc_session::ensure_phi_ready() {
-    if (!fuse_phi || phi_ready == next_iter) return;
-    set_base_kernel<<<1, 1, 0, stream>>>(iter_base, next_iter);
-    HIPCHECK(hipGetLastError());
-    HIPCHECK(ggdmc_internal_launch_phi(&d, 0, phi_lds, stream));
-    phi_ready = next_iter;
+/* Stores the subject states of the last iteration run (deferred in the
+ * kernels to the next iteration's first launch) when that column is due. */
+void ggdmc_session::flush_store() {
+    const uint32_t it = next_iter - 1;
+    if (it >= 1 && it % (uint32_t)cfg.thin == 0 && d.S > 0)
+        HIPCHECK(ggdmc_internal_launch_store(&d, it, 1, stream));
 }
 
 void ggdmc_session::capture(int K) {
@@ -620,7 +619,6 @@ void ggdmc_session::capture(int K) {
 
 void ggdmc_session::run_batched(uint32_t n_iter) {
     if (n_iter == 0) return;
-    ensure_phi_ready();
     while (n_iter > 0) {
         if (graph && n_iter >= (uint32_t)graph_iters) {
             set_base_kernel<<<1, 1, 0, stream>>>(iter_base, next_iter);
@@ -636,7 +634,7 @@ void ggdmc_session::run_batched(uint32_t n_iter) {
             n_iter -= 1;
         }
     }
-    phi_ready = next_iter;   /* fused: the last launch did the next population step */
+    flush_store();
 }
 
 void ggdmc_session::run_plain(uint32_t n_iter) {
@@ -657,7 +655,7 @@ void ggdmc_session::run_plain(uint32_t n_iter) {
             for (int sw = 0; sw < nsweep_sub; ++sw)
                 for (int j = 0; j < d.nchain; ++j)
                     HIPCHECK(ggdmc_internal_launch_plain_subject(&d, next_iter, sw, j, stream));
-        HIPCHECK(ggdmc_internal_launch_store(&d, next_iter, stream));
+        HIPCHECK(ggdmc_internal_launch_store(&d, next_iter, 0, stream));
     }
 }
 
@@ -711,7 +709,6 @@ void ggdmc_session::autotune() {
     };
     save();
     const uint32_t it0 = next_iter;
-    const uint32_t ready0 = phi_ready;
     float best = 1e30f;
     Cand pick = cands[0];
     const int K = 4;
@@ -728,7 +725,6 @@ void ggdmc_session::autotune() {
         HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
         drop_graph();
         restore();
-        phi_ready = ready0;
         if (ms < best) {
             best = ms;
             pick = c;
@@ -779,7 +775,8 @@ int ggdmc_session_create(const ggdmc_config *cfg, const ggdmc_model *model, cons
         s->build(cfg, model, jobs, njobs);
         if (s->mode != GGDMC_FP64_PLAIN) {
             /* tune the tile shape when the run is long enough to repay it */
-            if (cfg->trials_per_tile == 0 && s->last_iter >= 64) s->autotune();
+            if (cfg->trials_per_tile == 0 && cfg->chains_per_item == 0 && s->last_iter >= 64)
+                s->autotune();
             const uint32_t total = s->last_iter;
             const int K = (int)std::min<uint32_t>(64u, total);
             s->capture(K);
@@ -855,14 +852,13 @@ int ggdmc_session_profile(ggdmc_session *s, int32_t n_iter, double *subject_kern
         const uint32_t left = s->next_iter <= s->last_iter ? s->last_iter - s->next_iter + 1 : 0;
         const uint32_t n = std::min<uint32_t>((uint32_t)n_iter, left);
         std::vector<hipEvent_t> ev;
-        s->ensure_phi_ready();
         for (uint32_t k = 0; k < n; ++k) {
             set_base_kernel<<<1, 1, 0, s->stream>>>(s->iter_base, s->next_iter);
             HIPCHECK(hipGetLastError());
             s->enqueue_iteration(0, &ev);
             s->next_iter += 1;
         }
-        s->phi_ready = s->next_iter;
+        s->flush_store();
         const size_t q = ev.size();
         HIPCHECK(hipStreamSynchronize(s->stream));
         double tot = 0.0;

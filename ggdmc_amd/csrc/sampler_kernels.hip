@@ -6,8 +6,9 @@
  *                        (src/de.cpp:567-613 / :111-155), LBA log-likelihood as
  *                        canonical chunk sums, Metropolis accept (:81-108) by the
  *                        item itself or by the last-arriving tile; migration ring
- *                        (src/de.cpp:615-665 / :157-199) by the subject's first
- *                        item; thinned store by the subject's last item.
+ *                        (src/de.cpp:615-665 / :157-199) and the deferred
+ *                        thinned store of the previous iteration by the
+ *                        subject's first item.
  *  phi_kernel            per-(chain, parameter) sufficient statistics of the
  *                        subject states, then the population-level step
  *                        (src/de.cpp:385-465, :467-565) with sumloghlike
@@ -153,6 +154,37 @@ __device__ __forceinline__ double subject_prior_term(const Dev &d, const JobDev 
     return log_prior1(d.pr_dist[k], d.pr_p0[k], d.pr_p1[k], d.pr_lo[k], d.pr_up[k], x);
 }
 
+/* The same term from cached logarithms (log scale, log normaliser): for
+ * hierarchical runs from the population step's cache of the chain's phi,
+ * for fixed priors from the session's; same operations, same values. */
+__device__ __forceinline__ double subject_prior_term_cached(const Dev &d, const JobDev &job, int chain,
+                                                            int p, double x) {
+    if (x != x) return x;
+    if (job.kind == GGDMC_KIND_HIER) {
+        const size_t r = ((size_t)job.phi_index * d.nchain + chain) * d.P + p;
+        const double *ph = d.phi + ((size_t)job.phi_index * d.nchain + chain) * (2 * d.P);
+        const double mu = ph[p], sg = ph[d.P + p];
+        if (!(sg > 0.0)) return __builtin_nan("");
+        if (x < d.pr_lo[job.prior_off + p] || x > d.pr_up[job.prior_off + p]) return -__builtin_inf();
+        const double z = (x - mu) / sg;
+        return -0.5 * z * z - 0.918938533204672741780329736406 - d.pcache[2 * r] - d.pcache[2 * r + 1];
+    }
+    const int k = job.prior_off + p;
+    const int dist = d.pr_dist[k];
+    const double p0 = d.pr_p0[k], p1 = d.pr_p1[k];
+    if (dist == GGDMC_DIST_UNIF) {
+        if (x < p0 || x > p1) return -__builtin_inf();
+        return -d.scache[2 * k];
+    }
+    if (dist == GGDMC_DIST_TNORM) {
+        if (!(p1 > 0.0)) return __builtin_nan("");
+        if (x < d.pr_lo[k] || x > d.pr_up[k]) return -__builtin_inf();
+        const double z = (x - p0) / p1;
+        return -0.5 * z * z - 0.918938533204672741780329736406 - d.scache[2 * k] - d.scache[2 * k + 1];
+    }
+    return __builtin_nan("");
+}
+
 /* sumlogprior (src/de.cpp:144, :186) of x: terms added in parameter order. */
 __device__ double subject_logprior(const Dev &d, const JobDev &job, int chain, const double *x) {
     double s = 0.0;
@@ -226,7 +258,7 @@ __device__ void proposal_priors(const Dev &d, const JobDev &job, const SweepLds 
     const int P = d.P;
     for (int e = threadIdx.x; e < k * P; e += blockDim.x) {
         const int c = e / P, p = e - c * P;
-        L.tm[e] = subject_prior_term(d, job, i0 + c, p, L.prop[e]);
+        L.tm[e] = subject_prior_term_cached(d, job, i0 + c, p, L.prop[e]);
     }
     __syncthreads();
     for (int c = threadIdx.x; c < k; c += blockDim.x) {
@@ -415,33 +447,6 @@ __device__ void store_subject(const Dev &d, const double *th, const double *lp, 
 }
 
 /* ------------------------------------------------------------ sweep kernel */
-__device__ void fused_phi_tail(const Dev &d, int x, const JobDev &job, uint32_t iter,
-                               char *lds_raw, int *flag);
-
-/* End of subject x's iteration in the final launch: the last of its w items
- * stores the state (re-read from global memory when other items wrote it)
- * and, for a hierarchical run, joins the run's population-step countdown. */
-__device__ void finish_subject(const Dev &d, const SweepLds &L, const JobDev &job, int x, int w,
-                               uint32_t iter, bool fuse_phi, char *lds_raw) {
-    if (w > 1) {
-        __threadfence();
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const uint32_t old = atomicAdd(d.subj_ctr + x, 1u);
-            L.flag[2] = old == (uint32_t)(w - 1);
-        }
-        __syncthreads();
-        if (!L.flag[2]) return;
-        __threadfence();
-        if (threadIdx.x == 0) d.subj_ctr[x] = 0u;
-        const size_t nP = (size_t)d.nchain * d.P;
-        store_subject(d, d.theta + x * nP, d.lp + (size_t)x * d.nchain, d.ll + (size_t)x * d.nchain, x, iter);
-    } else {
-        store_subject(d, L.th, L.lp, L.ll, x, iter);
-    }
-    if (fuse_phi && job.kind == GGDMC_KIND_HIER) fused_phi_tail(d, x, job, iter, lds_raw, L.flag);
-}
-
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs a) {
     extern __shared__ __attribute__((aligned(16))) char lds_raw[];
@@ -449,7 +454,6 @@ __global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs 
     const int x = item.x;
     const int c0 = item.y >> 16, c1 = item.y & 0xFFFF;
     const int t = item.z >> 16, ntile = item.z & 0xFFFF;
-    const int w = item.w;
     const JobDev job = d.jobs[d.subj_job[x]];
     const int n = d.nchain, P = d.P, G = d.G;
     const SweepLds L = carve(lds_raw, n, P, a.kmax, d.n_cond * d.n_acc, a.chain_batch, a.cmax);
@@ -470,6 +474,8 @@ __global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs 
         L.ll[c] = d.ll[(size_t)x * n + c];
     }
     __syncthreads();
+    /* deferred store: the first launch of iteration i stores iteration i - 1 */
+    if (a.store_prev && c0 == 0 && t == 0 && iter > 1) store_subject(d, L.th, L.lp, L.ll, x, iter - 1);
 
     if (migrate) {
         const bool act = blocked && single ? a.sweep == 0 : true;
@@ -522,7 +528,6 @@ __global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs 
             accept_proposals(d, L, x, i0, k);
         }
     }
-    if (a.final_launch) finish_subject(d, L, job, x, w, iter, a.fuse_phi != 0, lds_raw);
 }
 
 /* ------------------------------------------------------------- phi level */
@@ -530,6 +535,8 @@ struct PhiLds {
     double *ph, *plp, *pll;  /* [n*2P], [n], [n]: population states */
     double *prop;            /* [ngmax*2P] proposals of a chain group */
     double *tm;              /* [(ngmax+1)*4P] prior and likelihood terms */
+    double *pc;              /* [n*P*2] log scale, log normaliser of every chain's phi */
+    double *pp;              /* [(ngmax+1)*P*2] the same for the proposals */
     double *U;               /* [ngmax] */
     int *m0, *m1, *acc;      /* [ngmax] */
     int *sub;                /* [n] */
@@ -538,7 +545,8 @@ struct PhiLds {
 
 __host__ __device__ inline size_t phi_lds_bytes(int n, int P, int G) {
     const size_t ng = (size_t)(n + G - 1) / G;
-    const size_t dbl = (size_t)n * 2 * P + 2 * (size_t)n + ng * 2 * P + (ng + 1) * 4 * P + ng;
+    const size_t dbl = (size_t)n * 2 * P + 2 * (size_t)n + ng * 2 * P + (ng + 1) * 4 * P +
+                       (size_t)n * P * 2 + (ng + 1) * P * 2 + ng;
     return dbl * sizeof(double) + (3 * ng + n + 4) * sizeof(int);
 }
 
@@ -551,6 +559,8 @@ __device__ PhiLds phi_carve(char *base, int n, int P, int G) {
     L.pll = p; p += n;
     L.prop = p; p += (size_t)ng * 2 * P;
     L.tm = p; p += (size_t)(ng + 1) * 4 * P;
+    L.pc = p; p += (size_t)n * P * 2;
+    L.pp = p; p += (size_t)(ng + 1) * P * 2;
     L.U = p; p += ng;
     int *q = reinterpret_cast<int *>(p);
     L.m0 = q; q += ng;
@@ -569,15 +579,23 @@ __device__ PhiLds phi_carve(char *base, int n, int P, int G) {
  * [lo, up]; summing the P shares in parameter order gives sumloghlike
  * (src/de.cpp:245-270). */
 __device__ double hll_term(const Dev &d, const JobDev &job, int p, double mu, double sg,
-                           const double *st, int S) {
+                           const double *lc, const double *st, int S) {
     const double mean = st[0], M2 = st[1], mn = st[2], mx = st[3];
     if (!(sg > 0.0) || mean != mean || mu != mu) return __builtin_nan("");
     const double lo = d.pr_lo[job.prior_off + p], up = d.pr_up[job.prior_off + p];
     if (mn < lo || mx > up) return -__builtin_inf();
-    const double Z = tnorm_mass((lo - mu) / sg, (up - mu) / sg);
     const double dm = mean - mu;
     return -(M2 + (double)S * dm * dm) / (2.0 * sg * sg) -
-           (double)S * (0.918938533204672741780329736406 + ::log(sg) + ::log(Z));
+           (double)S * (0.918938533204672741780329736406 + lc[0] + lc[1]);
+}
+
+/* lc = (log sg, log Z) of parameter p's truncated normal with location mu
+ * and scale sg on the subject prior's bounds (Z its mass). */
+__device__ __forceinline__ void phi_log_consts(const Dev &d, const JobDev &job, int p, double mu,
+                                               double sg, double *lc) {
+    const double lo = d.pr_lo[job.prior_off + p], up = d.pr_up[job.prior_off + p];
+    lc[0] = ::log(sg);
+    lc[1] = ::log(tnorm_mass((lo - mu) / sg, (up - mu) / sg));
 }
 
 __device__ __forceinline__ double hyper_prior_term(const Dev &d, const JobDev &job, int q, double x) {
@@ -706,12 +724,15 @@ __device__ void phi_migration(const Dev &d, const JobDev &job, const PhiLds &L, 
                 v = hyper_prior_term(d, job, e, L.prop[e]);
             } else if (e < 3 * P) {
                 const int p = e - P2;
-                v = hll_term(d, job, p, L.prop[p], L.prop[P + p], st_src + p * 4, S);
+                double *lc = L.pp + (size_t)p * 2;
+                phi_log_consts(d, job, p, L.prop[p], L.prop[P + p], lc);
+                v = hll_term(d, job, p, L.prop[p], L.prop[P + p], lc, st_src + p * 4, S);
             } else {
                 const int which = e < 4 * P ? src : dst;
                 const int p = e - (e < 4 * P ? 3 * P : 4 * P);
                 const double *ph = L.ph + (size_t)which * P2;
-                v = hll_term(d, job, p, ph[p], ph[P + p], stats + ((size_t)which * P + p) * 4, S);
+                v = hll_term(d, job, p, ph[p], ph[P + p], L.pc + ((size_t)which * P + p) * 2,
+                             stats + ((size_t)which * P + p) * 4, S);
             }
             L.tm[e] = v;
         }
@@ -737,8 +758,10 @@ __device__ void phi_migration(const Dev &d, const JobDev &job, const PhiLds &L, 
             }
         }
         __syncthreads();
-        if (L.flag[1])
+        if (L.flag[1]) {
             for (int q = threadIdx.x; q < P2; q += blockDim.x) L.ph[(size_t)dst * P2 + q] = L.prop[q];
+            for (int q = threadIdx.x; q < 2 * P; q += blockDim.x) L.pc[(size_t)dst * P * 2 + q] = L.pp[q];
+        }
         __syncthreads();
     }
 }
@@ -789,11 +812,16 @@ __device__ void phi_crossover(const Dev &d, const JobDev &job, const PhiLds &L, 
         double v;
         if (r < P2) {
             v = hyper_prior_term(d, job, r, x[r]);
+        } else if (r < 3 * P) {
+            const int p = r - P2;
+            double *lc = L.pp + ((size_t)c * P + p) * 2;
+            phi_log_consts(d, job, p, x[p], x[P + p], lc);
+            v = hll_term(d, job, p, x[p], x[P + p], lc, stats + ((size_t)(hier ? i : 0) * P + p) * 4, S);
         } else {
-            const int p = r < 3 * P ? r - P2 : r - 3 * P;
-            const double *sti = stats + ((size_t)(hier ? i : 0) * P + p) * 4;
-            const double *v2 = r < 3 * P ? x : L.ph + (size_t)i * P2;
-            v = hll_term(d, job, p, v2[p], v2[P + p], sti, S);
+            const int p = r - 3 * P;
+            const double *cur = L.ph + (size_t)i * P2;
+            v = hll_term(d, job, p, cur[p], cur[P + p], L.pc + ((size_t)i * P + p) * 2,
+                         stats + ((size_t)(hier ? i : 0) * P + p) * 4, S);
         }
         L.tm[(size_t)c * 4 * P + r] = v;
     }
@@ -820,7 +848,10 @@ __device__ void phi_crossover(const Dev &d, const JobDev &job, const PhiLds &L, 
     __syncthreads();
     for (int e = threadIdx.x; e < ng * P2; e += blockDim.x) {
         const int c = e / P2;
-        if (L.acc[c]) L.ph[(size_t)lo * P2 + e] = L.prop[e];
+        if (L.acc[c]) {
+            L.ph[(size_t)lo * P2 + e] = L.prop[e];
+            L.pc[(size_t)lo * P * 2 + e] = L.pp[e];   /* P2 == 2P entries per chain */
+        }
     }
     __syncthreads();
 }
@@ -835,6 +866,11 @@ __device__ void phi_update(const Dev &d, int f, const JobDev &job, uint32_t iter
     for (int c = threadIdx.x; c < n; c += blockDim.x) {
         L.plp[c] = d.phi_lp[(size_t)f * n + c];
         L.pll[c] = d.phi_ll[(size_t)f * n + c];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < n * P; e += blockDim.x) {
+        const int c = e / P, p = e - c * P;
+        phi_log_consts(d, job, p, L.ph[(size_t)c * P2 + p], L.ph[(size_t)c * P2 + P + p], L.pc + (size_t)e * 2);
     }
     __syncthreads();
     const uint32_t unit = hier ? kUnitPhi : 0u;
@@ -857,7 +893,10 @@ __device__ void phi_update(const Dev &d, int f, const JobDev &job, uint32_t iter
             phi_crossover(d, job, L, st, group_lo(g, n, G), group_lo(g + 1, n, G), blocked ? sw : -1,
                           stats, hier);
     }
-    for (int e = threadIdx.x; e < n * P2; e += blockDim.x) d.phi[(size_t)f * n * P2 + e] = L.ph[e];
+    for (int e = threadIdx.x; e < n * P2; e += blockDim.x) {
+        d.phi[(size_t)f * n * P2 + e] = L.ph[e];
+        d.pcache[(size_t)f * n * P2 + e] = L.pc[e];   /* n * P * 2 == n * P2 */
+    }
     for (int c = threadIdx.x; c < n; c += blockDim.x) {
         d.phi_lp[(size_t)f * n + c] = L.plp[c];
         d.phi_ll[(size_t)f * n + c] = L.pll[c];
@@ -909,36 +948,6 @@ __global__ __launch_bounds__(kBlock) void phi_kernel(Dev d, uint32_t iter_off, i
     combine_stats(job, 0, n * P, bst, d.stats + (size_t)f * n * P * 4);
     __syncthreads();
     phi_update(d, f, job, iter, L);
-}
-
-/* Called by the workgroup that finished subject x's iteration: the last
- * subject of the run to finish computes every chain's statistics and the
- * population step of the next iteration (all of the run's subjects are done,
- * so no workgroup still reads its population state). */
-__device__ void fused_phi_tail(const Dev &d, int x, const JobDev &job, uint32_t iter,
-                               char *lds_raw, int *flag) {
-    __threadfence();
-    __syncthreads();
-    uint32_t *ctr = d.job_ctr + d.subj_job[x];
-    if (threadIdx.x == 0) {
-        const uint32_t old = atomicAdd(ctr, 1u);
-        flag[0] = old == (uint32_t)(job.nsubject - 1);
-    }
-    __syncthreads();
-    const bool last = flag[0] != 0;
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    if (threadIdx.x == 0) *ctr = 0u;
-    if (iter + 1 > d.last_iter) return;
-    const int f = job.phi_index, n = d.nchain, P = d.P;
-    double *bst = d.bstats + (size_t)f * d.bstats_stride;
-    block_stats(d, job, 0, n * P * ((job.nsubject + kStatBlock - 1) / kStatBlock), bst);
-    __syncthreads();
-    combine_stats(job, 0, n * P, bst, d.stats + (size_t)f * n * P * 4);
-    __syncthreads();
-    const PhiLds L = phi_carve(lds_raw, n, P, d.G);
-    phi_update(d, f, job, iter + 1, L);
 }
 
 /* ------------------------------------------------------- plain FP64 path */
@@ -1136,8 +1145,10 @@ __global__ __launch_bounds__(kBlock) void plain_phi_kernel(Dev d, uint32_t iter,
     }
 }
 
-/* Thinned store of every subject and population state (plain path). */
-__global__ void store_kernel(Dev d, uint32_t iter) {
+/* Thinned store of every subject and (unless subjects_only) population
+ * state: the plain path after each iteration, the batched path for the last
+ * iteration of an advance (whose subject store is otherwise deferred). */
+__global__ void store_kernel(Dev d, uint32_t iter, int subjects_only) {
     if (iter % (uint32_t)d.thin != 0) return;
     const int k = (int)(iter / (uint32_t)d.thin);
     if (k >= d.nmc) return;
@@ -1153,6 +1164,7 @@ __global__ void store_kernel(Dev d, uint32_t iter) {
         d.out_lp[(s * d.nmc + k) * n + r] = d.lp[e];
         d.out_ll[(s * d.nmc + k) * n + r] = d.ll[e];
     }
+    if (subjects_only) return;
     const int P2 = 2 * P;
     for (size_t e = tid; e < (size_t)d.n_phi * n * P2; e += stride) {
         const size_t f = e / ((size_t)n * P2), r = e - f * n * P2;
@@ -1256,8 +1268,9 @@ hipError_t ggdmc_internal_launch_plain_phi(const Dev *d, uint32_t iter, int swee
     return hipGetLastError();
 }
 
-hipError_t ggdmc_internal_launch_store(const Dev *d, uint32_t iter, hipStream_t stream) {
-    store_kernel<<<dim3(256), dim3(kBlock), 0, stream>>>(*d, iter);
+hipError_t ggdmc_internal_launch_store(const Dev *d, uint32_t iter, int subjects_only,
+                                      hipStream_t stream) {
+    store_kernel<<<dim3(256), dim3(kBlock), 0, stream>>>(*d, iter, subjects_only);
     return hipGetLastError();
 }
 
