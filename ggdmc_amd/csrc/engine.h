@@ -77,6 +77,7 @@ struct Dev {
     int64_t bstats_stride;      /* doubles per run: n * P * blocks of the largest run * 4 */
     double *csum;
     double *pcache;             /* [F][n][P][2]: log scale, log mass of each chain's subject prior */
+    int32_t *pcache_valid;      /* [F]: pcache matches phi (cleared on start and restore) */
     const double *scache;       /* [prior entries][2]: the same for fixed priors (unif: log width) */
 
     uint32_t *stats_ctr;        /* [F] statistics workgroups finished */

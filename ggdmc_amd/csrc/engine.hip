@@ -422,6 +422,8 @@ void ggdmc_session::build(const ggdmc_config *c, const ggdmc_model *model, const
     d.phi_ll = pool.upload(pll0, stream);
     d.stats = pool.alloc<double>((size_t)std::max(F, 1) * n * P * 4);
     d.pcache = pool.alloc<double>((size_t)std::max(F, 1) * n * P * 2);
+    d.pcache_valid = pool.alloc<int32_t>((size_t)std::max(F, 1));
+    HIPCHECK(hipMemsetAsync(d.pcache_valid, 0, sizeof(int32_t) * std::max(F, 1), stream));
     {
         int max_sub = 1;
         for (const auto &j : jobs)
@@ -705,6 +707,8 @@ void ggdmc_session::autotune() {
         copy(d.phi_lp, snap + nth + 2 * nch + nph, npc * 8);
         copy(d.phi_ll, snap + nth + 2 * nch + nph + npc, npc * 8);
         copy(d.evals, ev_snap, nev * 8);
+        if (d.n_phi > 0)
+            HIPCHECK(hipMemsetAsync(d.pcache_valid, 0, sizeof(int32_t) * d.n_phi, stream));
         HIPCHECK(hipStreamSynchronize(stream));
     };
     save();

@@ -57,6 +57,21 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+/* Sum over the 8 lanes of an aligned lane group (fixed xor tree). */
+__device__ __forceinline__ double sum8(double v) {
+#pragma unroll
+    for (int off = 4; off > 0; off >>= 1) v += __shfl_xor(v, off, 8);
+    return v;
+}
+
+/* Terms t[0..m) summed by 8 lanes (lane l takes terms l, l + 8, ... in order,
+ * then the xor tree); every lane of the group gets the sum. */
+__device__ __forceinline__ double group8_sum(const double *t, int m, int l) {
+    double v = 0.0;
+    for (int q = l; q < m; q += 8) v += t[q];
+    return sum8(v);
+}
+
 /* Deterministic block sum (fixed tree); all threads get the result. */
 __device__ double block_sum(double v, double *red) {
     const int tid = threadIdx.x;
@@ -202,19 +217,24 @@ __device__ __forceinline__ bool mh_accept(double tpost, double cur, double U) {
 struct SweepLds {
     double *th, *lp, *ll;   /* [n*P], [n], [n]: the subject's chain states */
     double *prop, *tm;      /* [kmax*P]: proposals, their prior terms */
+    double *jit;            /* [kmax*P]: jitter of each proposal parameter */
+    double *pri;            /* [kmax*P*4]: prior location, scale, log scale, log mass */
+    double *bnd;            /* [P*2]: prior bounds */
     double *tlp, *U, *tll;  /* [kmax] */
     double *acck;           /* [chain_batch * per_chain] accumulator constants */
     double *csum;           /* [kmax * cmax] chunk sums */
     int *m0, *m1, *active, *acc; /* [kmax] */
     int *sub;               /* [n] migration subset */
+    int *pdist;             /* [P] prior distribution of each parameter */
     int *flag;              /* [4] */
 };
 
 __host__ __device__ inline size_t sweep_lds_bytes(int n, int P, int kmax, int per_chain,
                                                   int chain_batch, int cmax) {
-    const size_t dbl = (size_t)n * P + 2 * (size_t)n + 2 * (size_t)kmax * P + 3 * (size_t)kmax +
-                       (size_t)chain_batch * per_chain * kAccKDoubles + (size_t)kmax * cmax;
-    const size_t ints = 4 * (size_t)kmax + n + 4;
+    const size_t dbl = (size_t)n * P + 2 * (size_t)n + 7 * (size_t)kmax * P + 2 * (size_t)P +
+                       3 * (size_t)kmax + (size_t)chain_batch * per_chain * kAccKDoubles +
+                       (size_t)kmax * cmax;
+    const size_t ints = 4 * (size_t)kmax + n + P + 4;
     return dbl * sizeof(double) + ints * sizeof(int);
 }
 
@@ -227,6 +247,9 @@ __device__ SweepLds carve(char *base, int n, int P, int kmax, int per_chain, int
     L.ll = p; p += n;
     L.prop = p; p += (size_t)kmax * P;
     L.tm = p; p += (size_t)kmax * P;
+    L.jit = p; p += (size_t)kmax * P;
+    L.pri = p; p += (size_t)kmax * P * 4;
+    L.bnd = p; p += (size_t)P * 2;
     L.tlp = p; p += kmax;
     L.U = p; p += kmax;
     L.tll = p; p += kmax;
@@ -238,6 +261,7 @@ __device__ SweepLds carve(char *base, int n, int P, int kmax, int per_chain, int
     L.active = q; q += kmax;
     L.acc = q; q += kmax;
     L.sub = q; q += n;
+    L.pdist = q; q += P;
     L.flag = q;
     return L;
 }
@@ -270,43 +294,114 @@ __device__ void proposal_priors(const Dev &d, const JobDev &job, const SweepLds 
     __syncthreads();
 }
 
-/* DE proposals of chains i0 .. i0+k-1 of group [lo, hi) (src/de.cpp:583-597,
- * :119-135): partners and accept U per chain, the jitter of parameters
- * 4b..4b+3 from Philox block 1 + b per (chain, b), prior terms per
- * (chain, parameter). para >= 0: blocked update of that parameter only. */
-__device__ void crossover_proposals(const Dev &d, const JobDev &job, const SweepLds &L,
-                                    const Stream &st, int lo, int hi, int i0, int k, int para) {
+/* Crossover proposals of chains i0 .. i0+k-1 of group [lo, hi) (src/de.cpp:583-597,
+ * :119-135) in three stages separated by barriers:
+ *  1. draws and constants that do not depend on the states: partners, accept U
+ *     and jitter per (chain, Philox block); the prior's location, scale and
+ *     cached logarithms per (chain, parameter) and its bounds per parameter;
+ *  2. per (chain, parameter): the proposal value and its prior term;
+ *  3. per chain: the prior terms summed by 8 lanes; per (chain, condition,
+ *     accumulator): the likelihood constants when the chains fit one batch.
+ * para >= 0: blocked update of that parameter only. */
+__device__ void proposal_stage1(const Dev &d, const JobDev &job, const SweepLds &L, const Stream &st,
+                                int lo, int hi, int i0, int k, int para) {
     const int P = d.P, n = d.nchain;
-    for (int c = threadIdx.x; c < k; c += blockDim.x) {
-        int m0, m1;
-        double U;
-        partners(st, i0 + c, lo, hi, n, m0, m1, U);
-        L.m0[c] = m0;
-        L.m1[c] = m1;
-        L.U[c] = U;
-    }
-    __syncthreads();
     const int nb = para >= 0 ? 1 : (P + 3) >> 2;
     for (int e = threadIdx.x; e < k * nb; e += blockDim.x) {
         const int c = e / nb, b = e - c * nb;
         const int i = i0 + c;
-        const double *xi = L.th + (size_t)i * P;
-        const double *x0 = L.th + (size_t)L.m0[c] * P, *x1 = L.th + (size_t)L.m1[c] * P;
-        double *x = L.prop + (size_t)c * P;
+        if (b == 0) {
+            int m0, m1;
+            double U;
+            partners(st, i, lo, hi, n, m0, m1, U);
+            L.m0[c] = m0;
+            L.m1[c] = m1;
+            L.U[c] = U;
+        }
         const U4 r = st.block((uint32_t)i, 1u + (uint32_t)b);
         if (para >= 0) {
-            for (int p = 0; p < P; ++p) x[p] = xi[p];
-            x[para] += jitter_word(r.x[0], d.rp) + job.gamma * (x0[para] - x1[para]);
+            L.jit[c * P + para] = jitter_word(r.x[0], d.rp);
         } else {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int p = 4 * b + q;
-                if (p < P) x[p] = xi[p] + (jitter_word(r.x[q], d.rp) + job.gamma * (x0[p] - x1[p]));
+            for (int j = 0; j < 4; ++j) {
+                const int p = 4 * b + j;
+                if (p < P) L.jit[c * P + p] = jitter_word(r.x[j], d.rp);
             }
         }
     }
-    __syncthreads();
-    proposal_priors(d, job, L, i0, k);
+    const bool hier = job.kind == GGDMC_KIND_HIER;
+    for (int e = threadIdx.x; e < k * P; e += blockDim.x) {
+        const int c = e / P, p = e - c * P;
+        double *q = L.pri + (size_t)e * 4;
+        if (hier) {
+            const size_t r = ((size_t)job.phi_index * n + i0 + c) * P + p;
+            const double *ph = d.phi + ((size_t)job.phi_index * n + i0 + c) * (2 * P);
+            q[0] = ph[p];
+            q[1] = ph[P + p];
+            q[2] = d.pcache[2 * r];
+            q[3] = d.pcache[2 * r + 1];
+        } else {
+            const int kk = job.prior_off + p;
+            q[0] = d.pr_p0[kk];
+            q[1] = d.pr_p1[kk];
+            q[2] = d.scache[2 * kk];
+            q[3] = d.scache[2 * kk + 1];
+        }
+    }
+    for (int p = threadIdx.x; p < P; p += blockDim.x) {
+        L.bnd[2 * p] = d.pr_lo[job.prior_off + p];
+        L.bnd[2 * p + 1] = d.pr_up[job.prior_off + p];
+        L.pdist[p] = hier ? GGDMC_DIST_TNORM : d.pr_dist[job.prior_off + p];
+    }
+}
+
+/* Prior term of x from the staged constants q = (location or lower end,
+ * scale or upper end, log scale or log width, log mass): the operations of
+ * log_prior1 with the logarithms cached. */
+__device__ __forceinline__ double staged_prior_term(int dist, const double *q, double lo, double up,
+                                                    double x) {
+    if (x != x) return x;
+    if (dist == GGDMC_DIST_UNIF) {
+        if (x < q[0] || x > q[1]) return -__builtin_inf();
+        return -q[2];
+    }
+    if (dist == GGDMC_DIST_TNORM) {
+        if (!(q[1] > 0.0)) return __builtin_nan("");
+        if (x < lo || x > up) return -__builtin_inf();
+        const double z = (x - q[0]) / q[1];
+        return -0.5 * z * z - 0.918938533204672741780329736406 - q[2] - q[3];
+    }
+    return __builtin_nan("");
+}
+
+__device__ void proposal_stage2(const Dev &d, const JobDev &job, const SweepLds &L, int i0, int k,
+                                int para) {
+    const int P = d.P;
+    for (int e = threadIdx.x; e < k * P; e += blockDim.x) {
+        const int c = e / P, p = e - c * P;
+        const double xi = L.th[(size_t)(i0 + c) * P + p];
+        double x = xi;
+        if (para < 0 || p == para)
+            x = xi + (L.jit[e] + job.gamma * (L.th[(size_t)L.m0[c] * P + p] - L.th[(size_t)L.m1[c] * P + p]));
+        L.prop[e] = x;
+        L.tm[e] = staged_prior_term(L.pdist[p], L.pri + (size_t)e * 4, L.bnd[2 * p], L.bnd[2 * p + 1], x);
+    }
+}
+
+/* Returns whether the likelihood constants of all k chains were built. */
+__device__ bool proposal_stage3(const Dev &d, const SweepLds &L, int k, int chain_batch) {
+    const int P = d.P;
+    for (int e = threadIdx.x; e < ((k + 7) & ~7) * 8; e += blockDim.x) {
+        const int c = e >> 3, l = e & 7;
+        const double v = group8_sum(L.tm + (size_t)(c < k ? c : 0) * P, P, l);
+        if (c < k && l == 0) {
+            L.tlp[c] = v;
+            L.active[c] = v > -__builtin_inf();
+        }
+    }
+    const bool all = k <= chain_batch;
+    if (all) build_acck(d, L.prop, P, 0, k, L.acck);
+    return all;
 }
 
 /* Log-likelihood of proposals L.prop[0..k) over chunks [j_lo, j_lo + nj) of
@@ -316,13 +411,15 @@ __device__ void crossover_proposals(const Dev &d, const JobDev &job, const Sweep
 template <int MODE>
 __device__ void proposals_loglike(const Dev &d, const SweepLds &L, int k, int chain_batch,
                                   int cmax, int64_t t0, int64_t t_end, int j_lo, int nj,
-                                  bool to_global, int64_t cbase, int i0) {
+                                  bool to_global, int64_t cbase, int i0, bool prebuilt = false) {
     const int P = d.P, n = d.nchain;
     for (int c = threadIdx.x; c < k; c += blockDim.x) L.tll[c] = 0.0;
     for (int cb = 0; cb < k; cb += chain_batch) {
         const int kb = min(chain_batch, k - cb);
-        __syncthreads();
-        build_acck(d, L.prop, P, cb, kb, L.acck);
+        if (!(prebuilt && cb == 0)) {
+            __syncthreads();
+            build_acck(d, L.prop, P, cb, kb, L.acck);
+        }
         __syncthreads();
         for (int jb = 0; jb < nj; jb += cmax) {
             const int nbat = min(cmax, nj - jb);
@@ -473,6 +570,20 @@ __global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs 
         L.lp[c] = d.lp[(size_t)x * n + c];
         L.ll[c] = d.ll[(size_t)x * n + c];
     }
+    const int para = blocked ? a.sweep : -1;
+    const Stream st = make_stream(seed, iter + d.philox_offset, unit, (uint32_t)a.sweep);
+    auto group_k = [&](int g, int &lo, int &hi, int &i0) {
+        lo = group_lo(g, n, G);
+        hi = group_lo(g + 1, n, G);
+        i0 = lo + c0;
+        return min(c1 == kAllChains ? hi - lo : c1, hi - lo) - c0;
+    };
+    if (!migrate) {
+        /* stage 1 of the first group overlaps the state loads */
+        int lo, hi, i0;
+        const int k = group_k(a.g_begin, lo, hi, i0);
+        if (k > 0) proposal_stage1(d, job, L, st, lo, hi, i0, k, para);
+    }
     __syncthreads();
     /* deferred store: the first launch of iteration i stores iteration i - 1 */
     if (a.store_prev && c0 == 0 && t == 0 && iter > 1) store_subject(d, L.th, L.lp, L.ll, x, iter - 1);
@@ -484,21 +595,25 @@ __global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs 
             subject_migration<MODE>(d, L, job, x, sd, para, a.cmax);
         }
     } else {
-        const int para = blocked ? a.sweep : -1;
-        const Stream st = make_stream(seed, iter + d.philox_offset, unit, (uint32_t)a.sweep);
         const int64_t s0 = d.subj_off[x], s1 = d.subj_off[x + 1];
         const int64_t t_lo = ntile > 1 ? s0 + (int64_t)t * d.tpt : s0;
         const int64_t t_hi = ntile > 1 ? min(s1, t_lo + (int64_t)d.tpt) : s1;
         const int j_lo = (int)((t_lo - s0) / kChunk);
         const int nj = (int)((t_hi - t_lo + kChunk - 1) / kChunk);
         for (int g = a.g_begin; g < a.g_end; ++g) {
-            const int lo = group_lo(g, n, G), hi = group_lo(g + 1, n, G);
-            const int k = min(c1 == kAllChains ? hi - lo : c1, hi - lo) - c0;
+            int lo, hi, i0;
+            const int k = group_k(g, lo, hi, i0);
             if (k <= 0) continue;
-            const int i0 = lo + c0;
-            crossover_proposals(d, job, L, st, lo, hi, i0, k, para);
+            if (g != a.g_begin) {
+                proposal_stage1(d, job, L, st, lo, hi, i0, k, para);
+                __syncthreads();
+            }
+            proposal_stage2(d, job, L, i0, k, para);
+            __syncthreads();
+            const bool built = proposal_stage3(d, L, k, a.chain_batch);
+            __syncthreads();
             proposals_loglike<MODE>(d, L, k, a.chain_batch, a.cmax, s0, s1, j_lo, nj, ntile > 1,
-                                    d.chunk_off[x], i0);
+                                    d.chunk_off[x], i0, built);
             if (threadIdx.x == 0) {
                 int na = 0;
                 for (int c = 0; c < k; ++c) na += L.active[c];
@@ -602,6 +717,26 @@ __device__ __forceinline__ double hyper_prior_term(const Dev &d, const JobDev &j
     const int k = job.hprior_off + q;
     return log_prior1(d.pr_dist[k], d.pr_p0[k], d.pr_p1[k], d.pr_lo[k], d.pr_up[k], x);
 }
+
+/* The same from the session's cached logarithms of the fixed prior. */
+__device__ __forceinline__ double hyper_prior_term_cached(const Dev &d, const JobDev &job, int q, double x) {
+    if (x != x) return x;
+    const int k = job.hprior_off + q;
+    const double p0 = d.pr_p0[k], p1 = d.pr_p1[k];
+    if (d.pr_dist[k] == GGDMC_DIST_UNIF) {
+        if (x < p0 || x > p1) return -__builtin_inf();
+        return -d.scache[2 * k];
+    }
+    if (d.pr_dist[k] == GGDMC_DIST_TNORM) {
+        if (!(p1 > 0.0)) return __builtin_nan("");
+        if (x < d.pr_lo[k] || x > d.pr_up[k]) return -__builtin_inf();
+        const double z = (x - p0) / p1;
+        return -0.5 * z * z - 0.918938533204672741780329736406 - d.scache[2 * k] - d.scache[2 * k + 1];
+    }
+    return __builtin_nan("");
+}
+
+
 
 /* sumlogprior of a population state x (src/de.cpp:426, :518): 2P terms in order. */
 __device__ double hyper_logprior(const Dev &d, const JobDev &job, const double *x) {
@@ -721,7 +856,7 @@ __device__ void phi_migration(const Dev &d, const JobDev &job, const PhiLds &L, 
         for (int e = threadIdx.x; e < nterm; e += blockDim.x) {
             double v;
             if (e < P2) {
-                v = hyper_prior_term(d, job, e, L.prop[e]);
+                v = hyper_prior_term_cached(d, job, e, L.prop[e]);
             } else if (e < 3 * P) {
                 const int p = e - P2;
                 double *lc = L.pp + (size_t)p * 2;
@@ -737,24 +872,26 @@ __device__ void phi_migration(const Dev &d, const JobDev &job, const PhiLds &L, 
             L.tm[e] = v;
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            if (hier) {
-                double a = 0.0, b = 0.0;
-                for (int p = 0; p < P; ++p) a += L.tm[3 * P + p];
-                for (int p = 0; p < P; ++p) b += L.tm[4 * P + p];
-                L.pll[src] = a;
-                L.pll[dst] = b;
-            }
-            double tlp = 0.0, tll = 0.0;
-            for (int q = 0; q < P2; ++q) tlp += L.tm[q];
-            for (int p = 0; p < P; ++p) tll += L.tm[P2 + p];
-            const double cur = L.plp[dst] + L.pll[dst];
-            const U4 b0 = sd.block(slot, 0);
-            const bool a = mh_accept(tlp + tll, cur, u53(b0.x[0], b0.x[1]));
-            L.flag[1] = a;
-            if (a) {
-                L.plp[dst] = tlp;
-                L.pll[dst] = tll;
+        if (threadIdx.x < 8) {
+            /* sums by 8 lanes; the refreshed shares, then the accept, by lane 0 */
+            const int l = threadIdx.x;
+            const double tlp = group8_sum(L.tm, P2, l);
+            const double tll = group8_sum(L.tm + P2, P, l);
+            const double ra = hier ? group8_sum(L.tm + 3 * P, P, l) : 0.0;
+            const double rb = hier ? group8_sum(L.tm + 4 * P, P, l) : 0.0;
+            if (l == 0) {
+                if (hier) {
+                    L.pll[src] = ra;
+                    L.pll[dst] = rb;
+                }
+                const double cur = L.plp[dst] + L.pll[dst];
+                const U4 b0 = sd.block(slot, 0);
+                const bool a = mh_accept(tlp + tll, cur, u53(b0.x[0], b0.x[1]));
+                L.flag[1] = a;
+                if (a) {
+                    L.plp[dst] = tlp;
+                    L.pll[dst] = tll;
+                }
             }
         }
         __syncthreads();
@@ -773,21 +910,17 @@ __device__ void phi_migration(const Dev &d, const JobDev &job, const PhiLds &L, 
 __device__ void phi_crossover(const Dev &d, const JobDev &job, const PhiLds &L, const Stream &st,
                               int lo, int hi, int para, const double *stats, bool hier) {
     const int n = d.nchain, P = d.P, P2 = 2 * P, S = job.nsubject, ng = hi - lo;
-    for (int c = threadIdx.x; c < ng; c += blockDim.x) {
-        int m0, m1;
-        double U;
-        partners(st, lo + c, lo, hi, n, m0, m1, U);
-        L.m0[c] = m0;
-        L.m1[c] = m1;
-        L.U[c] = U;
-    }
-    __syncthreads();
+    /* partners, accept U and jitter: one thread per (chain, Philox block) */
     const int nb = para >= 0 ? 1 : (P2 + 3) >> 2;
     for (int e = threadIdx.x; e < ng * nb; e += blockDim.x) {
         const int c = e / nb, b = e - c * nb;
         const int i = lo + c;
+        int m0, m1;
+        double U;
+        partners(st, i, lo, hi, n, m0, m1, U);
+        if (b == 0) L.U[c] = U;
         const double *xi = L.ph + (size_t)i * P2;
-        const double *x0 = L.ph + (size_t)L.m0[c] * P2, *x1 = L.ph + (size_t)L.m1[c] * P2;
+        const double *x0 = L.ph + (size_t)m0 * P2, *x1 = L.ph + (size_t)m1 * P2;
         double *x = L.prop + (size_t)c * P2;
         const U4 r = st.block((uint32_t)i, 1u + (uint32_t)b);
         if (para >= 0) {
@@ -803,46 +936,50 @@ __device__ void phi_crossover(const Dev &d, const JobDev &job, const PhiLds &L, 
     }
     __syncthreads();
     /* per chain c, tm[c*4P + ...]: [0, 2P) prior, [2P, 3P) proposal shares,
-     * [3P, 4P) refreshed current shares */
-    const int nterm = hier ? 4 * P : 3 * P;
-    for (int e = threadIdx.x; e < ng * nterm; e += blockDim.x) {
-        const int c = e / nterm, r = e - c * nterm;
+     * [3P, 4P) refreshed current shares (src/de.cpp:397); one loop per kind so
+     * that a wave runs one kind of term */
+    for (int e = threadIdx.x; e < ng * P2; e += blockDim.x) {
+        const int c = e / P2, q = e - c * P2;
+        L.tm[(size_t)c * 4 * P + q] = hyper_prior_term_cached(d, job, q, L.prop[e]);
+    }
+    for (int e = threadIdx.x; e < ng * P; e += blockDim.x) {
+        const int c = e / P, p = e - c * P;
         const int i = lo + c;
         const double *x = L.prop + (size_t)c * P2;
-        double v;
-        if (r < P2) {
-            v = hyper_prior_term(d, job, r, x[r]);
-        } else if (r < 3 * P) {
-            const int p = r - P2;
-            double *lc = L.pp + ((size_t)c * P + p) * 2;
-            phi_log_consts(d, job, p, x[p], x[P + p], lc);
-            v = hll_term(d, job, p, x[p], x[P + p], lc, stats + ((size_t)(hier ? i : 0) * P + p) * 4, S);
-        } else {
-            const int p = r - 3 * P;
+        double *lc = L.pp + (size_t)e * 2;
+        phi_log_consts(d, job, p, x[p], x[P + p], lc);
+        L.tm[(size_t)c * 4 * P + P2 + p] =
+            hll_term(d, job, p, x[p], x[P + p], lc, stats + ((size_t)(hier ? i : 0) * P + p) * 4, S);
+    }
+    if (hier) {
+        for (int e = threadIdx.x; e < ng * P; e += blockDim.x) {
+            const int c = e / P, p = e - c * P;
+            const int i = lo + c;
             const double *cur = L.ph + (size_t)i * P2;
-            v = hll_term(d, job, p, cur[p], cur[P + p], L.pc + ((size_t)i * P + p) * 2,
-                         stats + ((size_t)(hier ? i : 0) * P + p) * 4, S);
+            L.tm[(size_t)c * 4 * P + 3 * P + p] = hll_term(d, job, p, cur[p], cur[P + p],
+                                                           L.pc + ((size_t)i * P + p) * 2,
+                                                           stats + ((size_t)i * P + p) * 4, S);
         }
-        L.tm[(size_t)c * 4 * P + r] = v;
     }
     __syncthreads();
-    for (int c = threadIdx.x; c < ng; c += blockDim.x) {
-        const int i = lo + c;
-        const double *tc = L.tm + (size_t)c * 4 * P;
-        double tlp = 0.0, tll = 0.0;
-        for (int q = 0; q < P2; ++q) tlp += tc[q];
-        for (int p = 0; p < P; ++p) tll += tc[P2 + p];
-        if (hier) {
-            double ref = 0.0;
-            for (int p = 0; p < P; ++p) ref += tc[3 * P + p];
-            L.pll[i] = ref;
-        }
-        const double cur = L.pll[i] + L.plp[i];
-        const bool a = mh_accept(tlp + tll, cur, L.U[c]);
-        L.acc[c] = a;
-        if (a) {
-            L.plp[i] = tlp;
-            L.pll[i] = tll;
+    /* sums by 8 lanes per chain, accept by the group's first lane */
+    for (int e = threadIdx.x; e < ((ng + 7) & ~7) * 8; e += blockDim.x) {
+        const int c = e >> 3, l = e & 7;
+        const bool on = c < ng;
+        const double *tc = L.tm + (size_t)(on ? c : 0) * 4 * P;
+        const double tlp = group8_sum(tc, P2, l);
+        const double tll = group8_sum(tc + P2, P, l);
+        const double ref = hier ? group8_sum(tc + 3 * P, P, l) : 0.0;
+        if (on && l == 0) {
+            const int i = lo + c;
+            if (hier) L.pll[i] = ref;
+            const double cur = L.pll[i] + L.plp[i];
+            const bool a = mh_accept(tlp + tll, cur, L.U[c]);
+            L.acc[c] = a;
+            if (a) {
+                L.plp[i] = tlp;
+                L.pll[i] = tll;
+            }
         }
     }
     __syncthreads();
@@ -868,9 +1005,18 @@ __device__ void phi_update(const Dev &d, int f, const JobDev &job, uint32_t iter
         L.pll[c] = d.phi_ll[(size_t)f * n + c];
     }
     __syncthreads();
+    /* log scale / log mass of every chain's subject prior: kept from the last
+     * population step, computed afresh after a start or restore */
+    const bool have = d.pcache_valid[f] != 0;
     for (int e = threadIdx.x; e < n * P; e += blockDim.x) {
         const int c = e / P, p = e - c * P;
-        phi_log_consts(d, job, p, L.ph[(size_t)c * P2 + p], L.ph[(size_t)c * P2 + P + p], L.pc + (size_t)e * 2);
+        if (have) {
+            L.pc[2 * e] = d.pcache[((size_t)f * n * P + e) * 2];
+            L.pc[2 * e + 1] = d.pcache[((size_t)f * n * P + e) * 2 + 1];
+        } else {
+            phi_log_consts(d, job, p, L.ph[(size_t)c * P2 + p], L.ph[(size_t)c * P2 + P + p],
+                           L.pc + (size_t)e * 2);
+        }
     }
     __syncthreads();
     const uint32_t unit = hier ? kUnitPhi : 0u;
@@ -901,6 +1047,7 @@ __device__ void phi_update(const Dev &d, int f, const JobDev &job, uint32_t iter
         d.phi_lp[(size_t)f * n + c] = L.plp[c];
         d.phi_ll[(size_t)f * n + c] = L.pll[c];
     }
+    if (threadIdx.x == 0) d.pcache_valid[f] = 1;
     if (iter % (uint32_t)d.thin == 0) {
         const int k = (int)(iter / (uint32_t)d.thin);
         if (k < d.nmc) {
