@@ -266,8 +266,10 @@ GGDMC_FN double naive_trial_term(const AccK *kc, int n_acc, int w, double rt) {
     double pdf, cdf;
     naive_pdf_cdf<Ops>(kc[w], rt - kc[w].t0, pdf, cdf);
     double dens = pdf != pdf ? pdf : (pdf > kFloor ? pdf : kFloor);
-    for (int j = 0; j < n_acc; ++j) {
-        if (j == w) continue;
+    /* losers in accumulator order; indexing the l-th loser (rather than
+     * skipping j == w) keeps every lane of a wave on the same iteration */
+    for (int l = 0; l + 1 < n_acc; ++l) {
+        const int j = l + (l >= w);
         naive_pdf_cdf<Ops>(kc[j], rt - kc[j].t0, pdf, cdf);
         const double F = cdf != cdf ? cdf : (cdf > kFloor ? cdf : kFloor);
         dens *= (1.0 - F);
@@ -284,8 +286,8 @@ GGDMC_FN double mixed_trial_term_flagged(const AccK *kc, int n_acc, int w, doubl
     double acc = 0.0, nanv = lp;
     int neg = 0;
     bool zero = false, nan = lp != lp;
-    for (int j = 0; j < n_acc; ++j) {
-        if (j == w) continue;
+    for (int l = 0; l + 1 < n_acc; ++l) {
+        const int j = l + (l >= w);
         const double S = mixed_loser_survivor<Ops>(kc[j], rt, ill);
         if (S != S) {
             nan = true;
