@@ -30,6 +30,7 @@
 namespace ggdmc {
 
 constexpr int kBlock = 256;        /* threads per workgroup of the sweep kernels */
+constexpr int kPhiBlock = 512;     /* threads per workgroup of the population kernel */
 constexpr int kWaves = kBlock / 64;
 constexpr int kMaxChains = 256;    /* chains per run */
 constexpr int kChunk = 128;        /* trials per canonical chunk sum */
@@ -105,6 +106,7 @@ struct SweepArgs {
     int32_t chain_batch;  /* chains whose accumulator constants fit in LDS at once */
     int32_t kmax;         /* most chains one item proposes */
     int32_t cmax;         /* chunk sums per chain held in LDS at once */
+    int32_t tstage;       /* trials per item staged in LDS (0: read from HBM) */
     int32_t n_items;
     const int4 *items;    /* grid = one workgroup per item */
 };

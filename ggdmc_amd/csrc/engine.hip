@@ -31,7 +31,8 @@
 using namespace ggdmc;
 
 extern "C" {
-size_t ggdmc_internal_sweep_lds(int n, int P, int kmax, int per_chain, int chain_batch, int cmax);
+size_t ggdmc_internal_sweep_lds(int n, int P, int kmax, int per_chain, int chain_batch, int cmax,
+                                int tstage);
 size_t ggdmc_internal_phi_lds(int n, int P, int G);
 hipError_t ggdmc_internal_launch_sweep(const Dev *d, const SweepArgs *a, int mode, size_t lds,
                                        hipStream_t stream);
@@ -117,7 +118,7 @@ struct ggdmc_session {
     bool fused = false;          /* one item per subject over every chain group */
     int kchain = 0;              /* chains per item */
     int tpt = 0;                 /* trials per item; 0 = whole subject */
-    int kmax = 1, cmax = 1, chain_batch = 1;
+    int kmax = 1, cmax = 1, chain_batch = 1, tstage = 0;
     std::vector<const int4 *> lists;   /* item list per launch (per group, or one) */
     std::vector<int> list_n;
     size_t sweep_lds = 0, phi_lds = 0;
@@ -553,12 +554,19 @@ void ggdmc_session::set_decomp(bool fused_new, int k_new, int tpt_new) {
     cmax = (int)std::min<int64_t>(max_chunks_item, std::max(1, 4096 / kmax));
     chain_batch = kmax;
     const int per_chain = d.n_cond * d.n_acc;
+    /* the item's trials in LDS when they fit in 24 KB */
+    int64_t max_item_trials = 0;
+    for (int x = 0; x < d.S; ++x)
+        max_item_trials = std::max<int64_t>(max_item_trials,
+                                            tpt == 0 ? subj_ntr[x] : std::min<int64_t>(tpt, subj_ntr[x]));
+    tstage = max_item_trials * 10 <= 24 * 1024 ? (int)max_item_trials : 0;
     auto bytes = [&]() {
-        return ggdmc_internal_sweep_lds(n, d.P, kmax, per_chain, chain_batch, cmax);
+        return ggdmc_internal_sweep_lds(n, d.P, kmax, per_chain, chain_batch, cmax, tstage);
     };
     const size_t soft = 48 * 1024, limit = (size_t)device_lds_limit();
     while (chain_batch > 1 && bytes() > soft) --chain_batch;
     while (cmax > 1 && bytes() > soft) cmax = (cmax + 1) / 2;
+    if (bytes() > soft) tstage = 0;
     sweep_lds = bytes();
     require(sweep_lds <= limit, "design too large for LDS (chains x parameters)");
     drop_graph();
@@ -577,7 +585,7 @@ void ggdmc_session::enqueue_iteration(uint32_t off, std::vector<hipEvent_t> *ev)
     for (int sw = 0; sw < nsweep_sub; ++sw) {
         for (int g = 0; g < ngl; ++g) {
             SweepArgs a{off, sw, fused ? 0 : g, fused ? d.G : g + 1, (int)(sw == 0 && g == 0), chain_batch,
-                        kmax, cmax, list_n[g], lists[g]};
+                        kmax, cmax, tstage, list_n[g], lists[g]};
             hipEvent_t e0 = nullptr, e1 = nullptr;
             if (ev) {
                 HIPCHECK(hipEventCreate(&e0));
@@ -1046,7 +1054,7 @@ int ggdmc_loglike(const ggdmc_model *model, const ggdmc_subject *data, const dou
         d.subj_off = reinterpret_cast<const int64_t *>(b + o_off);
         d.S = 1;
         const int cmax = (int)std::max<int64_t>(1, std::min<int64_t>((T + kChunk - 1) / kChunk, 1024));
-        const size_t lds = ggdmc_internal_sweep_lds(1, P, 1, nq * na, 1, cmax);
+        const size_t lds = ggdmc_internal_sweep_lds(1, P, 1, nq * na, 1, cmax, 0);
         HIPCHECK(ggdmc_internal_launch_loglike(&d, reinterpret_cast<const double *>(b + o_th), n, 0,
                                                precision, cmax, lds, nullptr,
                                                reinterpret_cast<double *>(b + o_out)));

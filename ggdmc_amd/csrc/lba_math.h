@@ -100,21 +100,18 @@ GGDMC_FN double exp2_neg(double y) {
     if (!(y > -1075.0)) return y == y ? 0.0 : y;
     const double n = std::rint(y);
     const double x = (y - n) * kLn2;
-    double p = 1.6059043836821613e-10;                 /* 1/13! */
-    p = p * x + 2.08767569878681e-09;                  /* 1/12! */
-    p = p * x + 2.505210838544172e-08;                 /* 1/11! */
-    p = p * x + 2.755731922398589e-07;                 /* 1/10! */
-    p = p * x + 2.7557319223985893e-06;                /* 1/9!  */
-    p = p * x + 2.48015873015873e-05;                  /* 1/8!  */
-    p = p * x + 0.0001984126984126984;                 /* 1/7!  */
-    p = p * x + 0.001388888888888889;                  /* 1/6!  */
-    p = p * x + 0.008333333333333333;                  /* 1/5!  */
-    p = p * x + 0.041666666666666664;                  /* 1/4!  */
-    p = p * x + 0.16666666666666666;                   /* 1/3!  */
-    p = p * x + 0.5;
-    p = p * x + 1.0;
-    p = p * x + 1.0;
-    return std::ldexp(p, (int)n);
+    /* Estrin's scheme: independent pairs, then powers of x (short chains) */
+    const double x2 = x * x, x4 = x2 * x2, x8 = x4 * x4;
+    const double a0 = 1.0 + x;                                        /* 1/0!, 1/1! */
+    const double a1 = 0.5 + 0.16666666666666666 * x;                  /* 1/2!, 1/3! */
+    const double a2 = 0.041666666666666664 + 0.008333333333333333 * x; /* 1/4!, 1/5! */
+    const double a3 = 0.001388888888888889 + 0.0001984126984126984 * x; /* 1/6!, 1/7! */
+    const double a4 = 2.48015873015873e-05 + 2.7557319223985893e-06 * x; /* 1/8!, 1/9! */
+    const double a5 = 2.755731922398589e-07 + 2.505210838544172e-08 * x; /* 1/10!, 1/11! */
+    const double a6 = 2.08767569878681e-09 + 1.6059043836821613e-10 * x; /* 1/12!, 1/13! */
+    const double b0 = a0 + a1 * x2, b1 = a2 + a3 * x2, b2 = a4 + a5 * x2;
+    const double c0 = b0 + b1 * x4, c1 = b2 + a6 * x4;
+    return std::ldexp(c0 + c1 * x8, (int)n);
 }
 
 /* e = exp(-z^2/2), qa = Q(|z|) = upper normal tail beyond |z|. */
@@ -123,9 +120,13 @@ GGDMC_FN void mixed_tail(double z, double &e, double &qa) {
     e = exp2_neg(-0.5 * z * z * kLog2e);
     const double a = std::fabs(z);
     const double t = rcp64<Ops>(1.0 + kMillsK * a);
-    double h = kMills[kMillsDeg];
-#pragma unroll
-    for (int k = kMillsDeg - 1; k >= 0; --k) h = h * t + kMills[k];
+    /* Estrin's scheme for sum_k c_k t^k, k = 0..11 */
+    const double t2 = t * t, t4 = t2 * t2, t8 = t4 * t4;
+    const double p0 = kMills[0] + kMills[1] * t, p1 = kMills[2] + kMills[3] * t;
+    const double p2 = kMills[4] + kMills[5] * t, p3 = kMills[6] + kMills[7] * t;
+    const double p4 = kMills[8] + kMills[9] * t, p5 = kMills[10] + kMills[11] * t;
+    const double q0 = p0 + p1 * t2, q1 = p2 + p3 * t2, q2 = p4 + p5 * t2;
+    const double h = (q0 + q1 * t4) + q2 * t8;
     qa = e * t * h;
 }
 

@@ -91,9 +91,17 @@ __device__ double block_sum(double v, double *red) {
  * (chain, chunk) unit at a time: lane l adds the terms of trials
  * t0 + j*kChunk + l and + 64 + l (zero past the end), then the fixed xor
  * butterfly. Chains with active[c] == 0 (prior density zero) get 0. */
+/* Trials of a work item: rt[i - origin], code[i - origin] for trial i (the
+ * item's trials staged in LDS, or the global arrays with origin 0). */
+struct TrialView {
+    const double *rt;
+    const uint16_t *code;
+    int64_t origin;
+};
+
 template <int MODE>
-__device__ void chunk_sums(const Dev &d, int64_t t0, int64_t t_end, int j_lo, int nj, int kb,
-                           const int *active, const double *acck, double *C, int c_stride) {
+__device__ void chunk_sums(const Dev &d, const TrialView &tv, int64_t t0, int64_t t_end, int j_lo, int nj,
+                           int kb, const int *active, const double *acck, double *C, int c_stride) {
     static_assert(kChunk == 128, "two trials per lane");
     const int lane = threadIdx.x & 63;
     /* the wave index is uniform: keep the unit loop scalar */
@@ -107,8 +115,8 @@ __device__ void chunk_sums(const Dev &d, int64_t t0, int64_t t_end, int j_lo, in
             const int64_t i1 = t0 + (int64_t)(j_lo + jj) * kChunk + lane, i2 = i1 + 64;
             double a = 0.0, b = 0.0;
             const bool h1 = i1 < t_end, h2 = i2 < t_end;
-            const int code1 = h1 ? d.tcode[i1] : 0, code2 = h2 ? d.tcode[i2] : 0;
-            const double rt1 = h1 ? d.rt[i1] : 0.0, rt2 = h2 ? d.rt[i2] : 0.0;
+            const int code1 = h1 ? tv.code[i1 - tv.origin] : 0, code2 = h2 ? tv.code[i2 - tv.origin] : 0;
+            const double rt1 = h1 ? tv.rt[i1 - tv.origin] : 0.0, rt2 = h2 ? tv.rt[i2 - tv.origin] : 0.0;
             const AccK *k1 = kc + (code1 >> 3) * d.n_acc, *k2 = kc + (code2 >> 3) * d.n_acc;
             if (MODE == MODE_MIXED) {
                 bool ill1 = false, ill2 = false;
@@ -220,6 +228,8 @@ struct SweepLds {
     double *jit;            /* [kmax*P]: jitter of each proposal parameter */
     double *pri;            /* [kmax*P*4]: prior location, scale, log scale, log mass */
     double *bnd;            /* [P*2]: prior bounds */
+    double *trt;            /* [tstage]: the item's response times */
+    uint16_t *tcd;          /* [tstage]: the item's trial codes */
     double *tlp, *U, *tll;  /* [kmax] */
     double *acck;           /* [chain_batch * per_chain] accumulator constants */
     double *csum;           /* [kmax * cmax] chunk sums */
@@ -230,16 +240,17 @@ struct SweepLds {
 };
 
 __host__ __device__ inline size_t sweep_lds_bytes(int n, int P, int kmax, int per_chain,
-                                                  int chain_batch, int cmax) {
+                                                  int chain_batch, int cmax, int tstage) {
     const size_t dbl = (size_t)n * P + 2 * (size_t)n + 7 * (size_t)kmax * P + 2 * (size_t)P +
                        3 * (size_t)kmax + (size_t)chain_batch * per_chain * kAccKDoubles +
                        (size_t)kmax * cmax;
     const size_t ints = 4 * (size_t)kmax + n + P + 4;
-    return dbl * sizeof(double) + ints * sizeof(int);
+    return (dbl + (size_t)tstage) * sizeof(double) + ints * sizeof(int) +
+           (((size_t)tstage * sizeof(uint16_t) + 7) & ~(size_t)7);
 }
 
 __device__ SweepLds carve(char *base, int n, int P, int kmax, int per_chain, int chain_batch,
-                          int cmax) {
+                          int cmax, int tstage) {
     SweepLds L;
     double *p = reinterpret_cast<double *>(base);
     L.th = p; p += (size_t)n * P;
@@ -255,6 +266,9 @@ __device__ SweepLds carve(char *base, int n, int P, int kmax, int per_chain, int
     L.tll = p; p += kmax;
     L.acck = p; p += (size_t)chain_batch * per_chain * kAccKDoubles;
     L.csum = p; p += (size_t)kmax * cmax;
+    L.trt = p; p += tstage;
+    L.tcd = reinterpret_cast<uint16_t *>(p);
+    p += ((size_t)tstage * sizeof(uint16_t) + 7) / 8;
     int *q = reinterpret_cast<int *>(p);
     L.m0 = q; q += kmax;
     L.m1 = q; q += kmax;
@@ -264,6 +278,26 @@ __device__ SweepLds carve(char *base, int n, int P, int kmax, int per_chain, int
     L.pdist = q; q += P;
     L.flag = q;
     return L;
+}
+
+/* dst[i] = src[i], i < count, by the whole workgroup with four loads in flight
+ * per thread (global -> LDS staging). */
+template <class T>
+__device__ __forceinline__ void stage_copy(T *dst, const T *src, int count) {
+    const int bs = blockDim.x;
+    for (int i0 = threadIdx.x; i0 < count; i0 += 4 * bs) {
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = i0 + u * bs;
+            v[u] = i < count ? src[i] : T();
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = i0 + u * bs;
+            if (i < count) dst[i] = v[u];
+        }
+    }
 }
 
 /* Fills the accumulator constants of proposals [cb, cb+kb). */
@@ -409,9 +443,9 @@ __device__ bool proposal_stage3(const Dev &d, const SweepLds &L, int k, int chai
  * d.csum[(cbase + j) * n + i0 + c]; otherwise their sequential sum goes to
  * L.tll[c]. */
 template <int MODE>
-__device__ void proposals_loglike(const Dev &d, const SweepLds &L, int k, int chain_batch,
-                                  int cmax, int64_t t0, int64_t t_end, int j_lo, int nj,
-                                  bool to_global, int64_t cbase, int i0, bool prebuilt = false) {
+__device__ void proposals_loglike(const Dev &d, const SweepLds &L, const TrialView &tv, int k,
+                                  int chain_batch, int cmax, int64_t t0, int64_t t_end, int j_lo,
+                                  int nj, bool to_global, int64_t cbase, int i0, bool prebuilt = false) {
     const int P = d.P, n = d.nchain;
     for (int c = threadIdx.x; c < k; c += blockDim.x) L.tll[c] = 0.0;
     for (int cb = 0; cb < k; cb += chain_batch) {
@@ -423,7 +457,7 @@ __device__ void proposals_loglike(const Dev &d, const SweepLds &L, int k, int ch
         __syncthreads();
         for (int jb = 0; jb < nj; jb += cmax) {
             const int nbat = min(cmax, nj - jb);
-            chunk_sums<MODE>(d, t0, t_end, j_lo + jb, nbat, kb, L.active + cb, L.acck, L.csum, cmax);
+            chunk_sums<MODE>(d, tv, t0, t_end, j_lo + jb, nbat, kb, L.active + cb, L.acck, L.csum, cmax);
             __syncthreads();
             if (to_global) {
                 for (int e = threadIdx.x; e < kb * nbat; e += blockDim.x) {
@@ -504,7 +538,7 @@ __device__ void subject_migration(const Dev &d, const SweepLds &L, const JobDev 
         }
         __syncthreads();
         proposal_priors(d, job, L, src, 1);
-        proposals_loglike<MODE>(d, L, 1, 1, cmax, t0, t1, 0, nj, false, 0, 0);
+        proposals_loglike<MODE>(d, L, TrialView{d.rt, d.tcode, 0}, 1, 1, cmax, t0, t1, 0, nj, false, 0, 0);
         if (threadIdx.x == 0) {
             if (L.active[0]) atomicAdd(d.evals + x, (unsigned long long)(t1 - t0));
             const double cur = L.lp[dst] + L.ll[dst];
@@ -553,7 +587,7 @@ __global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs 
     const int t = item.z >> 16, ntile = item.z & 0xFFFF;
     const JobDev job = d.jobs[d.subj_job[x]];
     const int n = d.nchain, P = d.P, G = d.G;
-    const SweepLds L = carve(lds_raw, n, P, a.kmax, d.n_cond * d.n_acc, a.chain_batch, a.cmax);
+    const SweepLds L = carve(lds_raw, n, P, a.kmax, d.n_cond * d.n_acc, a.chain_batch, a.cmax, a.tstage);
     const uint32_t iter = *d.iter_base + a.iter_off;
     const bool single = job.kind != GGDMC_KIND_HIER;
     const bool blocked = d.blocked_sub != 0;
@@ -565,10 +599,19 @@ __global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs 
     const Stream sd = make_stream(seed, iter + d.philox_offset, unit, (uint32_t)dsweep);
     const bool migrate = decision_u(sd) < d.sub_mig;
 
-    for (int e = threadIdx.x; e < n * P; e += blockDim.x) L.th[e] = d.theta[(size_t)x * n * P + e];
-    for (int c = threadIdx.x; c < n; c += blockDim.x) {
-        L.lp[c] = d.lp[(size_t)x * n + c];
-        L.ll[c] = d.ll[(size_t)x * n + c];
+    stage_copy(L.th, d.theta + (size_t)x * n * P, n * P);
+    stage_copy(L.lp, d.lp + (size_t)x * n, n);
+    stage_copy(L.ll, d.ll + (size_t)x * n, n);
+    /* the item's trials; staged in LDS when they fit */
+    const int64_t s0 = d.subj_off[x], s1 = d.subj_off[x + 1];
+    const int64_t t_lo = ntile > 1 ? s0 + (int64_t)t * d.tpt : s0;
+    const int64_t t_hi = ntile > 1 ? min(s1, t_lo + (int64_t)d.tpt) : s1;
+    const bool staged = a.tstage > 0 && !migrate;
+    TrialView tv{d.rt, d.tcode, 0};
+    if (staged) {
+        stage_copy(L.trt, d.rt + t_lo, (int)(t_hi - t_lo));
+        stage_copy(L.tcd, d.tcode + t_lo, (int)(t_hi - t_lo));
+        tv = TrialView{L.trt, L.tcd, t_lo};
     }
     const int para = blocked ? a.sweep : -1;
     const Stream st = make_stream(seed, iter + d.philox_offset, unit, (uint32_t)a.sweep);
@@ -595,9 +638,6 @@ __global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs 
             subject_migration<MODE>(d, L, job, x, sd, para, a.cmax);
         }
     } else {
-        const int64_t s0 = d.subj_off[x], s1 = d.subj_off[x + 1];
-        const int64_t t_lo = ntile > 1 ? s0 + (int64_t)t * d.tpt : s0;
-        const int64_t t_hi = ntile > 1 ? min(s1, t_lo + (int64_t)d.tpt) : s1;
         const int j_lo = (int)((t_lo - s0) / kChunk);
         const int nj = (int)((t_hi - t_lo + kChunk - 1) / kChunk);
         for (int g = a.g_begin; g < a.g_end; ++g) {
@@ -612,7 +652,7 @@ __global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs 
             __syncthreads();
             const bool built = proposal_stage3(d, L, k, a.chain_batch);
             __syncthreads();
-            proposals_loglike<MODE>(d, L, k, a.chain_batch, a.cmax, s0, s1, j_lo, nj, ntile > 1,
+            proposals_loglike<MODE>(d, L, tv, k, a.chain_batch, a.cmax, s0, s1, j_lo, nj, ntile > 1,
                                     d.chunk_off[x], i0, built);
             if (threadIdx.x == 0) {
                 int na = 0;
@@ -1066,7 +1106,7 @@ __device__ void phi_update(const Dev &d, int f, const JobDev &job, uint32_t iter
  * workgroup of each run combines them and performs the population step.
  * Hyper-only runs have fixed statistics: their workgroup b == 0 steps
  * directly. */
-__global__ __launch_bounds__(kBlock) void phi_kernel(Dev d, uint32_t iter_off, int nwg) {
+__global__ __launch_bounds__(kPhiBlock) void phi_kernel(Dev d, uint32_t iter_off, int nwg) {
     extern __shared__ __attribute__((aligned(16))) char lds_raw[];
     const int n = d.nchain, P = d.P;
     const int f = blockIdx.x / nwg, b = blockIdx.x - f * nwg;
@@ -1082,16 +1122,21 @@ __global__ __launch_bounds__(kBlock) void phi_kernel(Dev d, uint32_t iter_off, i
     double *bst = d.bstats + (size_t)f * d.bstats_stride;
     if (b * (int)blockDim.x < nitem)
         block_stats(d, job, b * (int)blockDim.x, min(nitem, (b + 1) * (int)blockDim.x), bst);
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t old = atomicAdd(d.stats_ctr + f, 1u);
-        L.flag[0] = old == (uint32_t)(nwg - 1);
+    if (nwg > 1) {
+        /* the last workgroup of the run continues */
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t old = atomicAdd(d.stats_ctr + f, 1u);
+            L.flag[0] = old == (uint32_t)(nwg - 1);
+        }
+        __syncthreads();
+        if (!L.flag[0]) return;
+        __threadfence();
+        if (threadIdx.x == 0) d.stats_ctr[f] = 0u;
+    } else {
+        __syncthreads();
     }
-    __syncthreads();
-    if (!L.flag[0]) return;
-    __threadfence();
-    if (threadIdx.x == 0) d.stats_ctr[f] = 0u;
     combine_stats(job, 0, n * P, bst, d.stats + (size_t)f * n * P * 4);
     __syncthreads();
     phi_update(d, f, job, iter, L);
@@ -1361,14 +1406,14 @@ __global__ __launch_bounds__(kBlock) void loglike_kernel(Dev d, const double *th
                                                         double *out) {
     extern __shared__ __attribute__((aligned(16))) char lds_raw[];
     const int P = d.P;
-    const SweepLds L = carve(lds_raw, 1, P, 1, d.n_cond * d.n_acc, 1, cmax);
+    const SweepLds L = carve(lds_raw, 1, P, 1, d.n_cond * d.n_acc, 1, cmax, 0);
     const int q = blockIdx.x;
     for (int p = threadIdx.x; p < P; p += blockDim.x) L.prop[p] = thetas[(size_t)q * P + p];
     if (threadIdx.x == 0) L.active[0] = 1;
     __syncthreads();
     const int64_t t0 = d.subj_off[s], t1 = d.subj_off[s + 1];
     const int nj = (int)((t1 - t0 + kChunk - 1) / kChunk);
-    proposals_loglike<MODE>(d, L, 1, 1, cmax, t0, t1, 0, nj, false, 0, 0);
+    proposals_loglike<MODE>(d, L, TrialView{d.rt, d.tcode, 0}, 1, 1, cmax, t0, t1, 0, nj, false, 0, 0);
     if (threadIdx.x == 0) out[q] = L.tll[0];
 }
 
@@ -1379,8 +1424,9 @@ using namespace ggdmc;
 
 extern "C" {
 
-size_t ggdmc_internal_sweep_lds(int n, int P, int kmax, int per_chain, int chain_batch, int cmax) {
-    return sweep_lds_bytes(n, P, kmax, per_chain, chain_batch, cmax);
+size_t ggdmc_internal_sweep_lds(int n, int P, int kmax, int per_chain, int chain_batch, int cmax,
+                                int tstage) {
+    return sweep_lds_bytes(n, P, kmax, per_chain, chain_batch, cmax, tstage);
 }
 size_t ggdmc_internal_phi_lds(int n, int P, int G) { return phi_lds_bytes(n, P, G); }
 
@@ -1396,9 +1442,9 @@ hipError_t ggdmc_internal_launch_sweep(const Dev *d, const SweepArgs *a, int mod
 
 hipError_t ggdmc_internal_launch_phi(const Dev *d, uint32_t iter_off, size_t lds, hipStream_t stream) {
     /* items of the largest run: pairs x subject blocks */
-    const int nwg = (int)((d->bstats_stride / 4 + kBlock - 1) / kBlock);
-    phi_kernel<<<dim3(d->n_phi * std::max(nwg, 1)), dim3(kBlock), lds, stream>>>(*d, iter_off,
-                                                                                std::max(nwg, 1));
+    const int nwg = (int)((d->bstats_stride / 4 + kPhiBlock - 1) / kPhiBlock);
+    phi_kernel<<<dim3(d->n_phi * std::max(nwg, 1)), dim3(kPhiBlock), lds, stream>>>(*d, iter_off,
+                                                                                   std::max(nwg, 1));
     return hipGetLastError();
 }
 
