@@ -67,6 +67,8 @@ struct Dev {
 
     const JobDev *jobs;
     const int32_t *phi_job;     /* [F] job of each population state */
+    const JobDev *subj_jobs;    /* [S] the job of each subject (one load from the subject) */
+    const JobDev *phi_jobs;     /* [F] the job of each population state */
 
     const int32_t *pr_dist;
     const double *pr_p0, *pr_p1, *pr_lo, *pr_up;

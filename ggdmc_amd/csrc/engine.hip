@@ -391,6 +391,13 @@ void ggdmc_session::build(const ggdmc_config *c, const ggdmc_model *model, const
     d.subj_unit = pool.upload(subj_unit, stream);
     d.jobs = pool.upload(jd, stream);
     d.phi_job = pool.upload(phi_job, stream);
+    {
+        std::vector<JobDev> sj, pj;
+        for (int32_t j : subj_job) sj.push_back(jd[j]);
+        for (int32_t j : phi_job) pj.push_back(jd[j]);
+        d.subj_jobs = pool.upload(sj, stream);
+        d.phi_jobs = pool.upload(pj, stream);
+    }
     d.pr_dist = pool.upload(pr_dist, stream);
     d.pr_p0 = pool.upload(pr_p0, stream);
     d.pr_p1 = pool.upload(pr_p1, stream);

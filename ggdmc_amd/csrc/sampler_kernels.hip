@@ -507,7 +507,7 @@ __device__ void accept_proposals(const Dev &d, const SweepLds &L, int x, int i0,
  * :157-199), in one workgroup; state updated in LDS and global memory. */
 template <int MODE>
 __device__ void subject_migration(const Dev &d, const SweepLds &L, const JobDev &job, int x,
-                                  const Stream &st, int para, int cmax) {
+                                  const Stream &st, int para, int cmax, const TrialView &tv) {
     const int n = d.nchain, P = d.P;
     if (threadIdx.x == 0) L.flag[0] = subchains(st, n, L.sub);
     __syncthreads();
@@ -538,7 +538,7 @@ __device__ void subject_migration(const Dev &d, const SweepLds &L, const JobDev 
         }
         __syncthreads();
         proposal_priors(d, job, L, src, 1);
-        proposals_loglike<MODE>(d, L, TrialView{d.rt, d.tcode, 0}, 1, 1, cmax, t0, t1, 0, nj, false, 0, 0);
+        proposals_loglike<MODE>(d, L, tv, 1, 1, cmax, t0, t1, 0, nj, false, 0, 0);
         if (threadIdx.x == 0) {
             if (L.active[0]) atomicAdd(d.evals + x, (unsigned long long)(t1 - t0));
             const double cur = L.lp[dst] + L.ll[dst];
@@ -585,9 +585,23 @@ __global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs 
     const int x = item.x;
     const int c0 = item.y >> 16, c1 = item.y & 0xFFFF;
     const int t = item.z >> 16, ntile = item.z & 0xFFFF;
-    const JobDev job = d.jobs[d.subj_job[x]];
     const int n = d.nchain, P = d.P, G = d.G;
     const SweepLds L = carve(lds_raw, n, P, a.kmax, d.n_cond * d.n_acc, a.chain_batch, a.cmax, a.tstage);
+    /* loads first (they depend on the item only), then the draws */
+    stage_copy(L.th, d.theta + (size_t)x * n * P, n * P);
+    stage_copy(L.lp, d.lp + (size_t)x * n, n);
+    stage_copy(L.ll, d.ll + (size_t)x * n, n);
+    /* the item's trials; staged in LDS when they fit */
+    const int64_t s0 = d.subj_off[x], s1 = d.subj_off[x + 1];
+    const int64_t t_lo = ntile > 1 ? s0 + (int64_t)t * d.tpt : s0;
+    const int64_t t_hi = ntile > 1 ? min(s1, t_lo + (int64_t)d.tpt) : s1;
+    TrialView tv{d.rt, d.tcode, 0};
+    if (a.tstage > 0) {
+        stage_copy(L.trt, d.rt + t_lo, (int)(t_hi - t_lo));
+        stage_copy(L.tcd, d.tcode + t_lo, (int)(t_hi - t_lo));
+        tv = TrialView{L.trt, L.tcd, t_lo};
+    }
+    const JobDev job = d.subj_jobs[x];
     const uint32_t iter = *d.iter_base + a.iter_off;
     const bool single = job.kind != GGDMC_KIND_HIER;
     const bool blocked = d.blocked_sub != 0;
@@ -598,21 +612,6 @@ __global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs 
     const int dsweep = (blocked && single) ? 0 : a.sweep;
     const Stream sd = make_stream(seed, iter + d.philox_offset, unit, (uint32_t)dsweep);
     const bool migrate = decision_u(sd) < d.sub_mig;
-
-    stage_copy(L.th, d.theta + (size_t)x * n * P, n * P);
-    stage_copy(L.lp, d.lp + (size_t)x * n, n);
-    stage_copy(L.ll, d.ll + (size_t)x * n, n);
-    /* the item's trials; staged in LDS when they fit */
-    const int64_t s0 = d.subj_off[x], s1 = d.subj_off[x + 1];
-    const int64_t t_lo = ntile > 1 ? s0 + (int64_t)t * d.tpt : s0;
-    const int64_t t_hi = ntile > 1 ? min(s1, t_lo + (int64_t)d.tpt) : s1;
-    const bool staged = a.tstage > 0 && !migrate;
-    TrialView tv{d.rt, d.tcode, 0};
-    if (staged) {
-        stage_copy(L.trt, d.rt + t_lo, (int)(t_hi - t_lo));
-        stage_copy(L.tcd, d.tcode + t_lo, (int)(t_hi - t_lo));
-        tv = TrialView{L.trt, L.tcd, t_lo};
-    }
     const int para = blocked ? a.sweep : -1;
     const Stream st = make_stream(seed, iter + d.philox_offset, unit, (uint32_t)a.sweep);
     auto group_k = [&](int g, int &lo, int &hi, int &i0) {
@@ -635,7 +634,8 @@ __global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs 
         const bool act = blocked && single ? a.sweep == 0 : true;
         if (c0 == 0 && t == 0 && act && a.g_begin == 0) {
             const int para = (blocked && !single) ? a.sweep : -1;
-            subject_migration<MODE>(d, L, job, x, sd, para, a.cmax);
+            subject_migration<MODE>(d, L, job, x, sd, para, a.cmax,
+                                    ntile == 1 && a.tstage > 0 ? tv : TrialView{d.rt, d.tcode, 0});
         }
     } else {
         const int j_lo = (int)((t_lo - s0) / kChunk);
@@ -1110,7 +1110,7 @@ __global__ __launch_bounds__(kPhiBlock) void phi_kernel(Dev d, uint32_t iter_off
     extern __shared__ __attribute__((aligned(16))) char lds_raw[];
     const int n = d.nchain, P = d.P;
     const int f = blockIdx.x / nwg, b = blockIdx.x - f * nwg;
-    const JobDev job = d.jobs[d.phi_job[f]];
+    const JobDev job = d.phi_jobs[f];
     const uint32_t iter = *d.iter_base + iter_off;
     const PhiLds L = phi_carve(lds_raw, n, P, d.G);
     if (job.kind != GGDMC_KIND_HIER) {
