@@ -55,6 +55,7 @@ struct Dev {
     uint32_t philox_offset;     /* Philox iteration = local iteration + this */
     uint32_t last_iter;         /* last local iteration of the run (nmc*thin - 1) */
 
+    const double *exp2tab;      /* [64] 2^(j/64) */
     const int32_t *par_src;     /* [n_cond][n_acc][6] */
     const double *par_const;
 

@@ -27,6 +27,7 @@
 
 #include "../../include/ggdmc_hip.h"
 #include "engine.h"
+#include "lba_math.h"
 
 using namespace ggdmc;
 
@@ -383,6 +384,11 @@ void ggdmc_session::build(const ggdmc_config *c, const ggdmc_model *model, const
     if (any_lba) {
         d.par_src = pool.upload(cond.src, stream);
         d.par_const = pool.upload(cond.cst, stream);
+    }
+    {
+        std::vector<double> tab(64);
+        fill_exp2_table(tab.data());
+        d.exp2tab = pool.upload(tab, stream);
     }
     d.rt = pool.upload(rt, stream);
     d.tcode = pool.upload(tcode, stream);
@@ -1036,7 +1042,8 @@ int ggdmc_loglike(const ggdmc_model *model, const ggdmc_subject *data, const dou
         const size_t o_rt = o_const + al(sizeof(double) * nq * na * 6);
         const size_t o_cell = o_rt + al(sizeof(double) * (size_t)T);
         const size_t o_off = o_cell + al(sizeof(uint16_t) * (size_t)T);
-        const size_t o_th = o_off + al(sizeof(soff));
+        const size_t o_tab = o_off + al(sizeof(soff));
+        const size_t o_th = o_tab + al(sizeof(double) * 64);
         const size_t o_out = o_th + al(sizeof(double) * (size_t)n * P);
         const size_t total = o_out + al(sizeof(double) * (size_t)n);
         std::vector<char> h(total);
@@ -1045,6 +1052,7 @@ int ggdmc_loglike(const ggdmc_model *model, const ggdmc_subject *data, const dou
         if (T) std::memcpy(&h[o_rt], data->rt, sizeof(double) * (size_t)T);
         if (T) std::memcpy(&h[o_cell], tcode.data(), sizeof(uint16_t) * (size_t)T);
         std::memcpy(&h[o_off], soff, sizeof(soff));
+        fill_exp2_table(reinterpret_cast<double *>(&h[o_tab]));
         std::memcpy(&h[o_th], theta, sizeof(double) * (size_t)n * P);
         HIPCHECK(hipMalloc(&dbuf, total));
         char *b = static_cast<char *>(dbuf);
@@ -1059,6 +1067,7 @@ int ggdmc_loglike(const ggdmc_model *model, const ggdmc_subject *data, const dou
         d.rt = reinterpret_cast<const double *>(b + o_rt);
         d.tcode = reinterpret_cast<const uint16_t *>(b + o_cell);
         d.subj_off = reinterpret_cast<const int64_t *>(b + o_off);
+        d.exp2tab = reinterpret_cast<const double *>(b + o_tab);
         d.S = 1;
         const int cmax = (int)std::max<int64_t>(1, std::min<int64_t>((T + kChunk - 1) / kChunk, 1024));
         const size_t lds = ggdmc_internal_sweep_lds(1, P, 1, nq * na, 1, cmax, 0);

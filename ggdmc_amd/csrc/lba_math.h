@@ -20,7 +20,8 @@
  *    non-packed FP32, so all polynomial / rational arithmetic stays FP64 and
  *    only the transcendental instructions run in FP32 hardware (v_exp_f32,
  *    v_log_f32, v_rcp_f32) with split arguments:
- *      * exp(-z^2/2) = 2^(y_hi) * (1 + y_lo*ln2), y = -z^2/2*log2(e) in FP64;
+ *      * exp(-z^2/2) = 2^y, y = -z^2/2*log2(e): 2^n * 2^(j/64) (table) * 2^f
+ *        (short polynomial), all FP64;
  *      * 1/x from v_rcp_f32 plus one FP64 Newton step;
  *      * Q(|z|) = exp(-z^2/2) * R(|z|), R the Mills-ratio factor
  *        Q(z)*exp(z^2/2), approximated as t*sum_k c_k t^k with t = 1/(1+0.275|z|)
@@ -93,31 +94,32 @@ GGDMC_FN double rcp64(double x) {
     return r * (2.0 - x * r);
 }
 
-/* 2^y for y <= 0 in FP64 arithmetic: y = n + f with n = rint(y), |f| <= 1/2,
- * and 2^f = exp(f ln 2) from its Taylor series to degree 13 (truncation below
- * 1e-17 for |f ln 2| <= 0.347). Underflows to 0 below 2^-1074; NaN propagates. */
+/* 2^y for y <= 0 in FP64 arithmetic: y = n + j/64 + f with integers n, j
+ * (0 <= j < 64) and |f| <= 1/128; 2^(j/64) from a table of correctly rounded
+ * values (Ops::exp2_frac), 2^f = exp(f ln 2) from its Taylor series to degree
+ * 5 (truncation below 4e-17 for |f ln 2| <= 0.0055). Underflows to 0 below
+ * 2^-1074; NaN propagates. */
+template <class Ops>
 GGDMC_FN double exp2_neg(double y) {
     if (!(y > -1075.0)) return y == y ? 0.0 : y;
-    const double n = std::rint(y);
-    const double x = (y - n) * kLn2;
-    /* Estrin's scheme: independent pairs, then powers of x (short chains) */
-    const double x2 = x * x, x4 = x2 * x2, x8 = x4 * x4;
-    const double a0 = 1.0 + x;                                        /* 1/0!, 1/1! */
-    const double a1 = 0.5 + 0.16666666666666666 * x;                  /* 1/2!, 1/3! */
-    const double a2 = 0.041666666666666664 + 0.008333333333333333 * x; /* 1/4!, 1/5! */
-    const double a3 = 0.001388888888888889 + 0.0001984126984126984 * x; /* 1/6!, 1/7! */
-    const double a4 = 2.48015873015873e-05 + 2.7557319223985893e-06 * x; /* 1/8!, 1/9! */
-    const double a5 = 2.755731922398589e-07 + 2.505210838544172e-08 * x; /* 1/10!, 1/11! */
-    const double a6 = 2.08767569878681e-09 + 1.6059043836821613e-10 * x; /* 1/12!, 1/13! */
-    const double b0 = a0 + a1 * x2, b1 = a2 + a3 * x2, b2 = a4 + a5 * x2;
-    const double c0 = b0 + b1 * x4, c1 = b2 + a6 * x4;
-    return std::ldexp(c0 + c1 * x8, (int)n);
+    const double k = std::rint(y * 64.0);
+    const double x = (y - k * 0.015625) * kLn2;
+    const int ki = (int)k;
+    const double x2 = x * x;
+    const double p = (1.0 + x) + x2 * ((0.5 + 0.16666666666666666 * x) +
+                                       x2 * (0.041666666666666664 + 0.008333333333333333 * x));
+    return std::ldexp(Ops::exp2_frac(ki & 63) * p, ki >> 6);
+}
+
+/* 2^(j/64), j = 0..63, correctly rounded to FP64 (the table of exp2_neg). */
+inline void fill_exp2_table(double *t) {
+    for (int j = 0; j < 64; ++j) t[j] = (double)std::exp2((long double)j / 64.0L);
 }
 
 /* e = exp(-z^2/2), qa = Q(|z|) = upper normal tail beyond |z|. */
 template <class Ops>
 GGDMC_FN void mixed_tail(double z, double &e, double &qa) {
-    e = exp2_neg(-0.5 * z * z * kLog2e);
+    e = exp2_neg<Ops>(-0.5 * z * z * kLog2e);
     const double a = std::fabs(z);
     const double t = rcp64<Ops>(1.0 + kMillsK * a);
     /* Estrin's scheme for sum_k c_k t^k, k = 0..11 */

@@ -26,7 +26,16 @@
 
 namespace ggdmc {
 
+/* 2^(j/64) for exp2_neg, staged into LDS by every kernel that evaluates the
+ * batched-path likelihood (load_exp2_table before its first barrier). */
+__shared__ double g_exp2tab[64];
+
+__device__ __forceinline__ void load_exp2_table(const Dev &d) {
+    for (int j = threadIdx.x; j < 64; j += blockDim.x) g_exp2tab[j] = d.exp2tab[j];
+}
+
 struct DevOps {
+    static __device__ __forceinline__ double exp2_frac(int j) { return g_exp2tab[j]; }
     static __device__ __forceinline__ float exp2(float x) { return __builtin_amdgcn_exp2f(x); }
     static __device__ __forceinline__ float log2(float x) { return __builtin_amdgcn_logf(x); }
     static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -587,6 +596,7 @@ __global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs 
     const int t = item.z >> 16, ntile = item.z & 0xFFFF;
     const int n = d.nchain, P = d.P, G = d.G;
     const SweepLds L = carve(lds_raw, n, P, a.kmax, d.n_cond * d.n_acc, a.chain_batch, a.cmax, a.tstage);
+    load_exp2_table(d);
     /* loads first (they depend on the item only), then the draws */
     stage_copy(L.th, d.theta + (size_t)x * n * P, n * P);
     stage_copy(L.lp, d.lp + (size_t)x * n, n);
@@ -1408,6 +1418,7 @@ __global__ __launch_bounds__(kBlock) void loglike_kernel(Dev d, const double *th
     const int P = d.P;
     const SweepLds L = carve(lds_raw, 1, P, 1, d.n_cond * d.n_acc, 1, cmax, 0);
     const int q = blockIdx.x;
+    load_exp2_table(d);
     for (int p = threadIdx.x; p < P; p += blockDim.x) L.prop[p] = thetas[(size_t)q * P + p];
     if (threadIdx.x == 0) L.active[0] = 1;
     __syncthreads();

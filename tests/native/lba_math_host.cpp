@@ -8,7 +8,14 @@
 #include <vector>
 
 namespace {
+struct Exp2Table {
+    double t[64];
+    Exp2Table() { ggdmc::fill_exp2_table(t); }
+};
+const Exp2Table g_exp2tab;
+
 struct HostOps {
+    static double exp2_frac(int j) { return g_exp2tab.t[j]; }
     static float exp2(float x) { return std::exp2(x); }
     static float log2(float x) { return std::log2(x); }
     static float rcp(float x) { return 1.0f / x; }
