@@ -101,14 +101,17 @@ GGDMC_FN double rcp64(double x) {
  * 2^-1074; NaN propagates. */
 template <class Ops>
 GGDMC_FN double exp2_neg(double y) {
-    if (!(y > -1075.0)) return y == y ? 0.0 : y;
-    const double k = std::rint(y * 64.0);
-    const double x = (y - k * 0.015625) * kLn2;
+    /* branch-free (selects), so that independent evaluations interleave */
+    const bool in = y > -1075.0;
+    const double yc = in ? y : 0.0;
+    const double k = std::rint(yc * 64.0);
+    const double x = (yc - k * 0.015625) * kLn2;
     const int ki = (int)k;
     const double x2 = x * x;
     const double p = (1.0 + x) + x2 * ((0.5 + 0.16666666666666666 * x) +
                                        x2 * (0.041666666666666664 + 0.008333333333333333 * x));
-    return std::ldexp(Ops::exp2_frac(ki & 63) * p, ki >> 6);
+    const double r = std::ldexp(Ops::exp2_frac(ki & 63) * p, ki >> 6);
+    return in ? r : (y == y ? 0.0 : y);
 }
 
 /* 2^(j/64), j = 0..63, correctly rounded to FP64 (the table of exp2_neg). */
@@ -137,10 +140,9 @@ template <class Ops>
 GGDMC_FN double mixed_log(double x) {
     int ex;
     double m = std::frexp(x, &ex); /* x = m * 2^ex, m in [0.5, 1) */
-    if (m < kSqrt1_2) {
-        m *= 2.0;
-        ex -= 1;
-    }
+    const bool lo = m < kSqrt1_2;
+    m = lo ? 2.0 * m : m;
+    ex = lo ? ex - 1 : ex;
     const float mf = (float)m;
     /* log(m) = log(mf) + r with r = (m - mf)/mf, |r| < 2^-24: an FP32 reciprocal suffices */
     const double r = (m - (double)mf) * (double)Ops::rcp(mf);
@@ -211,39 +213,36 @@ constexpr double kUlp1 = 1.1102230246251565e-16;
 /* Winner: log(max(pdf, 1e-10)); NaN propagates. */
 template <class Ops>
 GGDMC_FN double mixed_winner_logpdf(const AccK &k, double rt, bool &ill) {
+    /* branch-free: t <= 0 (or NaN) is evaluated at t = 1 and replaced */
     const double t = rt - k.t0;
-    if (!(t > 0.0)) return t == t ? kLogFloor : t;
-    const Core c = mixed_core<Ops>(k, t);
+    const bool pos = t > 0.0;
+    const Core c = mixed_core<Ops>(k, pos ? t : 1.0);
     const double num = k.v * c.dP + k.s * c.dphi;
     const double err = kTailErr * std::fabs(k.v) * (c.q1 + c.q2) + kExpErr * k.s * (c.e1 + c.e2) * kInvSqrt2Pi +
                        (c.z2 > 0.0 ? 2.0 * kUlp1 * std::fabs(k.v) : 0.0);
     const double pdf = num * k.invA * k.inv_den;
     /* both floored unless the pdf or its uncertainty reaches the floor */
-    if (err > kCondTol * std::fabs(num) && (std::fabs(num) + err) * k.invA * k.inv_den > 0.5 * kFloor)
-        ill = true;
-    return pdf < kFloor ? kLogFloor : mixed_log<Ops>(pdf);
+    ill |= pos && err > kCondTol * std::fabs(num) && (std::fabs(num) + err) * k.invA * k.inv_den > 0.5 * kFloor;
+    const double lp = pdf < kFloor ? kLogFloor : mixed_log<Ops>(pdf);
+    return pos ? lp : (t == t ? kLogFloor : t);
 }
 
 /* Loser: the reference's FP64 survivor 1 - max(F, 1e-10) (may be <= 0). */
 template <class Ops>
 GGDMC_FN double mixed_loser_survivor(const AccK &k, double rt, bool &ill) {
+    /* branch-free: t <= 0 (or NaN) is evaluated at t = 1 and replaced */
     const double t = rt - k.t0;
-    if (!(t > 0.0)) return t == t ? 1.0 - kFloor : t;
-    const Core c = mixed_core<Ops>(k, t);
+    const bool pos = t > 0.0;
+    const Core c = mixed_core<Ops>(k, pos ? t : 1.0);
     const double achi = std::fabs(c.chi), achiA = std::fabs(c.chi - k.A);
     const double G = (c.zs * c.dphi + c.chi * c.Q1 - (c.chi - k.A) * c.Q2) * k.invA;
-    double F;
-    if (G * k.inv_den < 0.5) {
-        F = G * k.inv_den;
-    } else {
-        /* 1 - G without cancellation, then the reference's own FP64 steps
-         * F = (1 + (G - 1)) / den: the rounding of 1 + (G - 1) is what
-         * quantises the reference's survivor, and it is reproduced exactly
-         * whenever 1 - G is accurate to well under half an ulp of 1. */
-        const double omG = (c.P2 - k.Pc) + (c.chi * c.dP - c.zs * c.dphi) * k.invA + k.Pc;
-        F = (1.0 - omG) / k.den;
-    }
-    const double S = 1.0 - (F > kFloor ? F : (F == F ? kFloor : F));
+    /* F >= 1/2: 1 - G without cancellation, then the reference's own FP64
+     * steps F = (1 + (G - 1)) / den: the rounding of 1 + (G - 1) is what
+     * quantises the reference's survivor, and it is reproduced exactly
+     * whenever 1 - G is accurate to well under half an ulp of 1 */
+    const double omG = (c.P2 - k.Pc) + (c.chi * c.dP - c.zs * c.dphi) * k.invA + k.Pc;
+    const double F = (G * k.inv_den < 0.5) ? G * k.inv_den : (1.0 - omG) / k.den;
+    const double S0 = 1.0 - (F > kFloor ? F : (F == F ? kFloor : F));
     /* error of F from the tails, plus the reference's own noise from Phi
      * values rounded near 1 (z > 0), against the survivor */
     const double errF = (kExpErr * c.zs * (c.e1 + c.e2) * kInvSqrt2Pi + kTailErr * (achi * c.q1 + achiA * c.q2) +
@@ -251,8 +250,8 @@ GGDMC_FN double mixed_loser_survivor(const AccK &k, double rt, bool &ill) {
                         k.invA * k.inv_den;
     /* below a hundredth of an ulp of 1 the reference's own rounding of F is
      * reproduced exactly, whatever the survivor */
-    if (errF > kCondTol * std::fabs(S) && errF > 1e-18) ill = true;
-    return S;
+    ill |= pos && errF > kCondTol * std::fabs(S0) && errF > 1e-18;
+    return pos ? S0 : (t == t ? 1.0 - kFloor : t);
 }
 
 /* Accumulates one survivor factor into (log |prod|, sign, zero) state. */
@@ -292,18 +291,14 @@ GGDMC_FN double mixed_trial_term_flagged(const AccK *kc, int n_acc, int w, doubl
     for (int l = 0; l + 1 < n_acc; ++l) {
         const int j = l + (l >= w);
         const double S = mixed_loser_survivor<Ops>(kc[j], rt, ill);
-        if (S != S) {
-            nan = true;
-            nanv = S;
-        }
+        const bool sn = S != S;
+        nan |= sn;
+        nanv = sn ? S : nanv;
         const double aS = std::fabs(S);
-        fold_survivor(S, aS > 0.0 ? mixed_log<Ops>(aS) : 0.0, acc, neg, zero);
+        fold_survivor(S, mixed_log<Ops>(aS > 0.0 ? aS : 1.0), acc, neg, zero);
     }
-    if (nan) {
-        ill = false;
-        return nanv;
-    }
-    return (zero || neg) ? kLogDblEps : lp + acc;
+    ill = ill && !nan;
+    return nan ? nanv : ((zero || neg) ? kLogDblEps : lp + acc);
 }
 
 /* Batched-path trial term: mixed evaluation, or the reference formula when

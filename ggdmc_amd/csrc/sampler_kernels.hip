@@ -124,16 +124,22 @@ __device__ void chunk_sums(const Dev &d, const TrialView &tv, int64_t t0, int64_
             const int64_t i1 = t0 + (int64_t)(j_lo + jj) * kChunk + lane, i2 = i1 + 64;
             double a = 0.0, b = 0.0;
             const bool h1 = i1 < t_end, h2 = i2 < t_end;
-            const int code1 = h1 ? tv.code[i1 - tv.origin] : 0, code2 = h2 ? tv.code[i2 - tv.origin] : 0;
-            const double rt1 = h1 ? tv.rt[i1 - tv.origin] : 0.0, rt2 = h2 ? tv.rt[i2 - tv.origin] : 0.0;
+            /* lanes past the end evaluate the chunk's first trial and drop it,
+             * so that both evaluations are straight-line code */
+            const int64_t r1 = (h1 ? i1 : t0 + (int64_t)(j_lo + jj) * kChunk) - tv.origin;
+            const int64_t r2 = (h2 ? i2 : t0 + (int64_t)(j_lo + jj) * kChunk) - tv.origin;
+            const int code1 = tv.code[r1], code2 = tv.code[r2];
+            const double rt1 = tv.rt[r1], rt2 = tv.rt[r2];
             const AccK *k1 = kc + (code1 >> 3) * d.n_acc, *k2 = kc + (code2 >> 3) * d.n_acc;
             if (MODE == MODE_MIXED) {
                 bool ill1 = false, ill2 = false;
-                if (h1) a = mixed_trial_term_flagged<DevOps>(k1, d.n_acc, code1 & 7, rt1, ill1);
-                if (h2) b = mixed_trial_term_flagged<DevOps>(k2, d.n_acc, code2 & 7, rt2, ill2);
+                const double ma = mixed_trial_term_flagged<DevOps>(k1, d.n_acc, code1 & 7, rt1, ill1);
+                const double mb = mixed_trial_term_flagged<DevOps>(k2, d.n_acc, code2 & 7, rt2, ill2);
+                a = h1 ? ma : 0.0;
+                b = h2 ? mb : 0.0;
                 /* rare: ill-conditioned trials take the reference formula */
-                if (ill1) a = naive_trial_term<DevOps>(k1, d.n_acc, code1 & 7, rt1);
-                if (ill2) b = naive_trial_term<DevOps>(k2, d.n_acc, code2 & 7, rt2);
+                if (h1 && ill1) a = naive_trial_term<DevOps>(k1, d.n_acc, code1 & 7, rt1);
+                if (h2 && ill2) b = naive_trial_term<DevOps>(k2, d.n_acc, code2 & 7, rt2);
             } else {
                 if (h1) a = naive_trial_term<DevOps>(k1, d.n_acc, code1 & 7, rt1);
                 if (h2) b = naive_trial_term<DevOps>(k2, d.n_acc, code2 & 7, rt2);
