@@ -6,12 +6,15 @@ from the LBA generation rule. A step is one full DE-MCMC iteration (population
 step + every subject's crossover / migration over all its trials).
 
 Measured on the same inputs, all resident in HBM before the timed region:
-  value          iterations/s of the batched path (FP32 transcendentals, FP64
-                 sums and sampler state, hipGraph-replayed launches)
+  value          iterations/s of the batched path (mixed FP32/FP64 density
+                 arithmetic with an error-model fallback to the FP64 formula,
+                 FP64 sums and sampler state, hipGraph-replayed launches)
   plain_fp64     iterations/s of the plain FP64 GPU path (the reference's
                  arithmetic and sequential chain order, one launch per chain step)
   max_rel_dLL    max |LL_batched / LL_fp64 - 1| over every stored subject state
-  roofline       of the subject sweep kernel (dominant), VALU issue bound
+  roofline       of the subject sweep kernel (dominant): VALU issue bound,
+                 algorithmic slots per evaluation x evaluations per launch /
+                 launch time (HIP events on the launch stream)
   cpu_baseline   the FP64 oracle restatement of the reference sampler, one core
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3]
@@ -30,11 +33,22 @@ sys.path.insert(0, ROOT)
 METRIC = ("DE-MCMC iter/s + trial-LL evals/s: batched FP32 vs plain FP64 GPU, "
           "|ΔLL|≤1e-6")
 
-# VALU issue slots per trial-likelihood evaluation of the batched kernel
-# (one slot = one lane for one VALU cycle; quarter-rate transcendentals count
-# 4), measured with rocprofv3 SQ_INSTS_VALU / SQ_INSTS_VALU_TRANS_F32 on the
-# subject sweep kernel; see profiles/.
-W_SLOTS = {2: 400.0, 4: 800.0}
+# Algorithmic VALU issue slots of one trial-likelihood evaluation (one trial
+# x one proposal, all accumulators; a slot = one lane for one VALU cycle, a
+# quarter-rate transcendental counts 4): per accumulator the shifted time,
+# scale, two z-scores, two exponentials, two normal tails and the winner
+# density or loser survivor, then the product, log and accumulate. About
+# 86 arithmetic + 11 transcendental instructions with 2 accumulators,
+# 170 + 21 with 4.
+W_ALG = {2: 130.0, 4: 255.0}
+# VALU lane-slots the subject sweep kernel actually issues per evaluation,
+# everything included (precision work, indexing, staging, sums, proposals):
+# rocprofv3 SQ_INSTS_VALU x 64 / evaluations of a launch (profiles/), at the
+# decomposition the autotuner picks there (2 chains x all trials per item).
+W_ISSUED = {"c3": 677.0}
+# HBM bytes per launch of the subject sweep kernel: rocprofv3 FETCH_SIZE x 2.0
+# + WRITE_SIZE x 1.0 (factors of a 256 MiB copy kernel; profiles/).
+TRAFFIC = {"c3": 4.128e6}
 # 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
 VALU_PEAK_TSLOTS = 256 * 4 * 16 * 2.4e9 / 1e12
 
@@ -73,18 +87,18 @@ def calibrate_copy():
 
 
 def cpu_baseline(w, budget_s):
-    """Oracle (FP64, single thread) on the same workload for a bounded number
-    of iterations; returns iterations/s and the sample description."""
+    """Oracle (FP64, single thread) on the same workload, every job of it one
+    after the other, for a bounded number of iterations; returns iterations/s
+    and the sample description."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     from ggdmc_amd import _lib
-    j = w.jobs[0]
     om = O.Model(w.model.winner, w.model.par_src, w.model.par_const, w.model.npar)
 
     def opr(p):
         return O.Prior(p.dist, p.p0, p.p1, p.lower, p.upper)
 
-    def one(n_iter):
+    def one(j, n_iter):
         cfg = O.config(nmc=n_iter + 1, nchain=w.nchain, nparameter_de=j.nparameter_de,
                        sub_migration_prob=w.sub_migration_prob,
                        pop_migration_prob=w.pop_migration_prob, seed=j.seed)
@@ -102,9 +116,12 @@ def cpu_baseline(w, budget_s):
             O.run_subject(cfg, om, opr(j.p_prior), j.subject.rt, j.subject.cell_off, *j.start)
         return time.perf_counter() - t
 
-    t1 = one(1)
+    def all_jobs(n_iter):
+        return sum(one(j, n_iter) for j in w.jobs)
+
+    t1 = all_jobs(1)
     n = max(2, int(budget_s / max(t1, 1e-6)))
-    dt = one(n)
+    dt = all_jobs(n)
     return n / dt, n, dt
 
 
@@ -159,14 +176,27 @@ def main():
     k_ms, n_launch = sess.profile(a.profile_iters)
     e3 = sess.trial_evals()
     evals_per_launch = (e3 - e2) / max(n_launch, 1)
-    achieved = evals_per_launch * W_SLOTS[n_acc] / (k_ms / 1e3) / 1e12
+    achieved = evals_per_launch * W_ALG[n_acc] / (k_ms / 1e3) / 1e12
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TSLOTS, 2),
                 "unit": "Tslot/s", "frac": round(achieved / VALU_PEAK_TSLOTS, 4),
-                "traffic": None, "kernel": "subject_sweep_kernel",
+                "traffic": TRAFFIC.get(w.name), "kernel": "subject_sweep_kernel",
                 "kernel_ms": round(k_ms, 4), "evals_per_launch": evals_per_launch,
-                "slots_per_eval": W_SLOTS[n_acc]}
+                "slots_per_eval": W_ALG[n_acc]}
+    wi = W_ISSUED.get(w.name)
+    if wi:
+        issued = evals_per_launch * wi / (k_ms / 1e3) / 1e12
+        roofline.update({"issued_slots_per_eval": wi, "issued": round(issued, 3),
+                         "issued_frac": round(issued / VALU_PEAK_TSLOTS, 4)})
+    if roofline["traffic"]:
+        roofline["hbm_GBps"] = round(roofline["traffic"] / (k_ms / 1e3) / 1e9, 1)
+    t_dl = time.perf_counter()
     res = sess.download()
+    dl_ms = (time.perf_counter() - t_dl) * 1e3
+    dl_bytes = sum(int(np.asarray(p.theta).nbytes + np.asarray(p.log_likelihoods).nbytes * 2)
+                   for r in res for p in ([r["phi"]] if r["phi"] is not None else []) + r["subjects"])
     sess.close()
+    print("[bench] samples download: %.1f MB in %.1f ms (%d stored columns)"
+          % (dl_bytes / 1e6, dl_ms, nmc), file=sys.stderr, flush=True)
 
     # per-proposal |dLL|: the last stored state of every chain against the
     # FP64 path (every subject for up to 64 subjects, an even subsample
@@ -201,15 +231,15 @@ def main():
     if not a.no_cpu:
         cps, n_cpu, dt = cpu_baseline(w, a.cpu_seconds)
         cpu = {"value": round(cps, 4), "unit": "iter/s", "cores": 1, "kind": "port",
-               "sample": "%d iterations of the same %s workload in %.1f s (FP64 oracle, "
-                         "one thread)" % (n_cpu, w.name, dt)}
+               "sample": "%d iterations of the same %s workload (all %d runs) in %.1f s "
+                         "(FP64 oracle, one thread)" % (n_cpu, w.name, len(w.jobs), dt)}
         print("[bench] cpu oracle: %s" % cpu["sample"], file=sys.stderr, flush=True)
 
     out = {
         "metric": METRIC, "value": round(iters_per_s, 2), "unit": "iter/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms / a.steps, 5),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32 transcendentals + fp64 sums/state", "data": "synthetic",
+        "dtype": "fp32/fp64 mixed (fp64 exp, sums, state)", "data": "synthetic",
         "config": {"workload": "%s: %s" % (w.name, w.description), "subjects": w.n_subjects,
                    "trials": w.n_trials, "chains": w.nchain, "sweep_groups": 2,
                    "chains_per_item": kci, "trials_per_item": tpt, "items_per_launch": nitems},
@@ -218,6 +248,7 @@ def main():
         "speedup_vs_plain_fp64": round(iters_per_s / plain, 2) if plain else None,
         "speedup_vs_cpu": round(iters_per_s / cpu["value"], 1) if cpu else None,
         "max_rel_dLL": worst, "dLL_states_checked": n_cmp,
+        "download": {"ms": round(dl_ms, 2), "MB": round(dl_bytes / 1e6, 2), "columns": nmc},
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out))
