@@ -81,11 +81,13 @@ struct Dev {
     int64_t bstats_stride;      /* doubles per run: n * P * blocks of the largest run * 4 */
     double *csum;
     double *pcache;             /* [F][n][P][2]: log scale, log mass of each chain's subject prior */
-    int32_t *pcache_valid;      /* [F]: pcache matches phi (cleared on start and restore) */
+    int32_t *pcache_valid;      /* [F][n]: pcache of a chain matches its phi (cleared on start and restore) */
     const double *scache;       /* [prior entries][2]: the same for fixed priors (unif: log width) */
 
     uint32_t *stats_ctr;        /* [F] statistics workgroups finished */
     uint32_t *tile_ctr;         /* [items] tiles of a (subject, chain range) finished */
+    uint32_t *phi_ready;        /* [F][n] last iteration whose population step of a chain is stored */
+    uint32_t *fault;            /* [1] nonzero: a wait for phi_ready gave up (results invalid) */
     unsigned long long *evals;  /* [S] trial-likelihood evaluations per subject */
     const uint32_t *iter_base;  /* device iteration base (graph replays) */
 
@@ -111,7 +113,10 @@ struct SweepArgs {
     int32_t cmax;         /* chunk sums per chain held in LDS at once */
     int32_t tstage;       /* trials per item staged in LDS (0: read from HBM) */
     int32_t n_items;
-    const int4 *items;    /* grid = one workgroup per item */
+    const int4 *items;    /* grid = phi_wgs + one workgroup per item */
+    int32_t phi_wgs;      /* leading workgroups that run the population step of the launch's
+                             chains, one per (state, chain) (0: none) */
+    int32_t phi_stats_lds; /* a whole-run step keeps the statistics in LDS */
 };
 
 } // namespace ggdmc

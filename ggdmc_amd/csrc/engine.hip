@@ -34,10 +34,12 @@ using namespace ggdmc;
 extern "C" {
 size_t ggdmc_internal_sweep_lds(int n, int P, int kmax, int per_chain, int chain_batch, int cmax,
                                 int tstage);
-size_t ggdmc_internal_phi_lds(int n, int P, int G);
+size_t ggdmc_internal_phi_lds(int n, int P, int G, int stats_lds);
+size_t ggdmc_internal_phi_chain_lds(int P, int max_subjects);
 hipError_t ggdmc_internal_launch_sweep(const Dev *d, const SweepArgs *a, int mode, size_t lds,
                                        hipStream_t stream);
-hipError_t ggdmc_internal_launch_phi(const Dev *d, uint32_t iter_off, size_t lds, hipStream_t stream);
+hipError_t ggdmc_internal_launch_phi(const Dev *d, uint32_t iter_off, size_t lds, int stats_lds,
+                                     hipStream_t stream);
 hipError_t ggdmc_internal_launch_plain_subject(const Dev *d, uint32_t iter, int sweep, int j,
                                                hipStream_t stream);
 hipError_t ggdmc_internal_launch_plain_phi(const Dev *d, uint32_t iter, int sweep, int j,
@@ -123,6 +125,13 @@ struct ggdmc_session {
     std::vector<const int4 *> lists;   /* item list per launch (per group, or one) */
     std::vector<int> list_n;
     size_t sweep_lds = 0, phi_lds = 0;
+    int phi_stats_lds = 0;        /* population kernel keeps the statistics in LDS */
+    /* population step inside the subject launches (leading workgroups of
+     * kBlock threads, one per (state, chain) of the launch's group) instead
+     * of its own kernel: */
+    bool overlap = false;
+    int ovl_stats_lds = 0;
+    size_t ovl_lds = 0;           /* LDS of those launches: max(sweep, population) */
     int nsweep_sub = 1;
     int64_t total_chunks = 0;
     uint32_t next_iter = 1;      /* next local iteration to run (1-based) */
@@ -158,6 +167,12 @@ struct ggdmc_session {
     void run_plain(uint32_t n_iter);
     void autotune();
     void capture(int K);
+    /* a subject workgroup gave up waiting for the population step */
+    void check_fault() {
+        uint32_t f = 0;
+        HIPCHECK(hipMemcpy(&f, d.fault, sizeof(f), hipMemcpyDeviceToHost));
+        if (f != 0) throw Error(GGDMC_EHIP, "population step not published in time: results invalid");
+    }
     void drop_graph() {
         if (graph) (void)hipGraphExecDestroy(graph);
         graph = nullptr;
@@ -436,8 +451,8 @@ void ggdmc_session::build(const ggdmc_config *c, const ggdmc_model *model, const
     d.phi_ll = pool.upload(pll0, stream);
     d.stats = pool.alloc<double>((size_t)std::max(F, 1) * n * P * 4);
     d.pcache = pool.alloc<double>((size_t)std::max(F, 1) * n * P * 2);
-    d.pcache_valid = pool.alloc<int32_t>((size_t)std::max(F, 1));
-    HIPCHECK(hipMemsetAsync(d.pcache_valid, 0, sizeof(int32_t) * std::max(F, 1), stream));
+    d.pcache_valid = pool.alloc<int32_t>((size_t)std::max(F, 1) * n);
+    HIPCHECK(hipMemsetAsync(d.pcache_valid, 0, sizeof(int32_t) * std::max(F, 1) * n, stream));
     {
         int max_sub = 1;
         for (const auto &j : jobs)
@@ -447,6 +462,10 @@ void ggdmc_session::build(const ggdmc_config *c, const ggdmc_model *model, const
     }
     d.stats_ctr = pool.alloc<uint32_t>((size_t)std::max(F, 1));
     HIPCHECK(hipMemsetAsync(d.stats_ctr, 0, sizeof(uint32_t) * std::max(F, 1), stream));
+    d.phi_ready = pool.alloc<uint32_t>((size_t)std::max(F, 1) * n);
+    HIPCHECK(hipMemsetAsync(d.phi_ready, 0, sizeof(uint32_t) * std::max(F, 1) * n, stream));
+    d.fault = pool.alloc<uint32_t>(1);
+    HIPCHECK(hipMemsetAsync(d.fault, 0, sizeof(uint32_t), stream));
     d.evals = pool.alloc<unsigned long long>((size_t)std::max(S, 1));
     HIPCHECK(hipMemsetAsync(d.evals, 0, sizeof(unsigned long long) * std::max(S, 1), stream));
     {
@@ -495,7 +514,8 @@ void ggdmc_session::build(const ggdmc_config *c, const ggdmc_model *model, const
     last_iter = (uint32_t)(cfg.nmc * cfg.thin - 1);
     d.last_iter = last_iter;
     next_iter = 1;
-    phi_lds = ggdmc_internal_phi_lds(n, P, G);
+    phi_stats_lds = ggdmc_internal_phi_lds(n, P, G, 1) <= (size_t)device_lds_limit() ? 1 : 0;
+    phi_lds = ggdmc_internal_phi_lds(n, P, G, phi_stats_lds);
     require(F == 0 || phi_lds <= (size_t)device_lds_limit(), "population state too large for LDS");
     default_decomp();
     HIPCHECK(hipStreamSynchronize(stream));
@@ -582,6 +602,17 @@ void ggdmc_session::set_decomp(bool fused_new, int k_new, int tpt_new) {
     if (bytes() > soft) tstage = 0;
     sweep_lds = bytes();
     require(sweep_lds <= limit, "design too large for LDS (chains x parameters)");
+    /* the population step of each chain group runs in that group's subject
+     * launch (not with one item per subject over every group, nor with
+     * blocked subject sweeps: those keep the population kernel) */
+    overlap = false;
+    if (d.n_phi > 0 && d.S > 0 && !fused && nsweep_sub == 1) {
+        const int max_sub = (int)(d.bstats_stride / 4 / ((int64_t)n * d.P)) * kStatBlock;
+        ovl_stats_lds = ggdmc_internal_phi_lds(n, d.P, G, 1) <= limit ? 1 : 0;
+        ovl_lds = std::max({sweep_lds, ggdmc_internal_phi_lds(n, d.P, G, ovl_stats_lds),
+                            ggdmc_internal_phi_chain_lds(d.P, max_sub)});
+        overlap = ovl_lds <= limit;
+    }
     drop_graph();
 }
 
@@ -591,14 +622,18 @@ void ggdmc_session::set_decomp(bool fused_new, int k_new, int tpt_new) {
  * first subject launch stores the previous iteration's subject states when
  * due. ev, if given, receives an event pair around every subject launch. */
 void ggdmc_session::enqueue_iteration(uint32_t off, std::vector<hipEvent_t> *ev) {
-    if (d.n_phi > 0) HIPCHECK(ggdmc_internal_launch_phi(&d, off, phi_lds, stream));
+    if (d.n_phi > 0 && !overlap) HIPCHECK(ggdmc_internal_launch_phi(&d, off, phi_lds, phi_stats_lds, stream));
     if (d.S == 0) return;
     const int lk = mode == GGDMC_FP64_BATCHED ? GGDMC_FP64_BATCHED : GGDMC_MIXED_BATCHED;
     const int ngl = fused ? 1 : d.G;
     for (int sw = 0; sw < nsweep_sub; ++sw) {
         for (int g = 0; g < ngl; ++g) {
-            SweepArgs a{off, sw, fused ? 0 : g, fused ? d.G : g + 1, (int)(sw == 0 && g == 0), chain_batch,
-                        kmax, cmax, tstage, list_n[g], lists[g]};
+            const bool first = sw == 0 && g == 0;
+            const bool with_phi = overlap && sw == 0;
+            const int nc = ((g + 1) * d.nchain) / d.G - (g * d.nchain) / d.G;   /* chains of group g */
+            SweepArgs a{off, sw, fused ? 0 : g, fused ? d.G : g + 1, (int)first, chain_batch,
+                        kmax, cmax, tstage, list_n[g], lists[g],
+                        with_phi ? d.n_phi * nc : 0, ovl_stats_lds};
             hipEvent_t e0 = nullptr, e1 = nullptr;
             if (ev) {
                 HIPCHECK(hipEventCreate(&e0));
@@ -607,7 +642,7 @@ void ggdmc_session::enqueue_iteration(uint32_t off, std::vector<hipEvent_t> *ev)
                 ev->push_back(e1);
                 HIPCHECK(hipEventRecord(e0, stream));
             }
-            HIPCHECK(ggdmc_internal_launch_sweep(&d, &a, lk, sweep_lds, stream));
+            HIPCHECK(ggdmc_internal_launch_sweep(&d, &a, lk, with_phi ? ovl_lds : sweep_lds, stream));
             if (ev) HIPCHECK(hipEventRecord(e1, stream));
         }
     }
@@ -728,8 +763,12 @@ void ggdmc_session::autotune() {
         copy(d.phi_lp, snap + nth + 2 * nch + nph, npc * 8);
         copy(d.phi_ll, snap + nth + 2 * nch + nph + npc, npc * 8);
         copy(d.evals, ev_snap, nev * 8);
-        if (d.n_phi > 0)
-            HIPCHECK(hipMemsetAsync(d.pcache_valid, 0, sizeof(int32_t) * d.n_phi, stream));
+        if (d.n_phi > 0) {
+            const size_t nf = (size_t)d.n_phi * d.nchain;
+            HIPCHECK(hipMemsetAsync(d.pcache_valid, 0, sizeof(int32_t) * nf, stream));
+            /* the iterations run again: their population steps are not done */
+            HIPCHECK(hipMemsetAsync(d.phi_ready, 0, sizeof(uint32_t) * nf, stream));
+        }
         HIPCHECK(hipStreamSynchronize(stream));
     };
     save();
@@ -828,6 +867,7 @@ int ggdmc_session_advance(ggdmc_session *s, int32_t n_iter) {
         HIPCHECK(hipEventRecord(s->ev1, s->stream));
         HIPCHECK(hipEventSynchronize(s->ev1));
         HIPCHECK(hipEventElapsedTime(&s->last_ms, s->ev0, s->ev1));
+        s->check_fault();
     } catch (const std::exception &e) {
         return fail(e);
     }
@@ -886,6 +926,7 @@ int ggdmc_session_profile(ggdmc_session *s, int32_t n_iter, double *subject_kern
         s->flush_store();
         const size_t q = ev.size();
         HIPCHECK(hipStreamSynchronize(s->stream));
+        s->check_fault();
         double tot = 0.0;
         for (size_t i = 0; i < q; i += 2) {
             float ms = 0.f;

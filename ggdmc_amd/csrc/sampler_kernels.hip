@@ -109,7 +109,7 @@ struct TrialView {
 };
 
 template <int MODE>
-__device__ void chunk_sums(const Dev &d, const TrialView &tv, int64_t t0, int64_t t_end, int j_lo, int nj,
+__device__ __forceinline__ void chunk_sums(const Dev &d, const TrialView &tv, int64_t t0, int64_t t_end, int j_lo, int nj,
                            int kb, const int *active, const double *acck, double *C, int c_stride) {
     static_assert(kChunk == 128, "two trials per lane");
     const int lane = threadIdx.x & 63;
@@ -264,7 +264,7 @@ __host__ __device__ inline size_t sweep_lds_bytes(int n, int P, int kmax, int pe
            (((size_t)tstage * sizeof(uint16_t) + 7) & ~(size_t)7);
 }
 
-__device__ SweepLds carve(char *base, int n, int P, int kmax, int per_chain, int chain_batch,
+__device__ __forceinline__ SweepLds carve(char *base, int n, int P, int kmax, int per_chain, int chain_batch,
                           int cmax, int tstage) {
     SweepLds L;
     double *p = reinterpret_cast<double *>(base);
@@ -295,18 +295,16 @@ __device__ SweepLds carve(char *base, int n, int P, int kmax, int per_chain, int
     return L;
 }
 
-/* dst[i] = src[i], i < count, by the whole workgroup with four loads in flight
- * per thread (global -> LDS staging). */
+/* dst[i] = src[i], start <= i < count, by the whole workgroup with four
+ * loads in flight per thread (global -> LDS staging). Loads use clamped
+ * indices, so no branch separates them. */
 template <class T>
-__device__ __forceinline__ void stage_copy(T *dst, const T *src, int count) {
+__device__ __forceinline__ void stage_copy(T *dst, const T *src, int count, int start = 0) {
     const int bs = blockDim.x;
-    for (int i0 = threadIdx.x; i0 < count; i0 += 4 * bs) {
+    for (int i0 = start + (int)threadIdx.x; i0 < count; i0 += 4 * bs) {
         T v[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int i = i0 + u * bs;
-            v[u] = i < count ? src[i] : T();
-        }
+        for (int u = 0; u < 4; ++u) v[u] = src[min(i0 + u * bs, count - 1)];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int i = i0 + u * bs;
@@ -315,8 +313,35 @@ __device__ __forceinline__ void stage_copy(T *dst, const T *src, int count) {
     }
 }
 
+/* One array of a multi-array staging step: load() issues this thread's
+ * first U elements (clamped indices, no branches), store() writes them and
+ * copies any elements beyond U * blockDim. Calling every piece's load()
+ * before any store() keeps all of a thread's loads in flight together, so
+ * the step costs one memory latency however many arrays it stages. */
+template <class T, int U>
+struct Piece {
+    T *dst;
+    const T *src;
+    int count;
+    T v[U];
+    __device__ __forceinline__ void load() {
+        if (count <= 0) return;
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = src[min((int)threadIdx.x + u * (int)blockDim.x, count - 1)];
+    }
+    __device__ __forceinline__ void store() {
+        if (count <= 0) return;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = (int)threadIdx.x + u * (int)blockDim.x;
+            if (i < count) dst[i] = v[u];
+        }
+        if (count > U * (int)blockDim.x) stage_copy(dst, src, count, U * (int)blockDim.x);
+    }
+};
+
 /* Fills the accumulator constants of proposals [cb, cb+kb). */
-__device__ void build_acck(const Dev &d, const double *prop, int P, int cb, int kb, double *acck) {
+__device__ __forceinline__ void build_acck(const Dev &d, const double *prop, int P, int cb, int kb, double *acck) {
     const int per_chain = d.n_cond * d.n_acc;
     AccK *K = reinterpret_cast<AccK *>(acck);
     for (int e = threadIdx.x; e < kb * per_chain; e += blockDim.x) {
@@ -327,7 +352,7 @@ __device__ void build_acck(const Dev &d, const double *prop, int P, int cb, int 
 
 /* Prior terms of proposals L.prop[0..k) for chains i0 + c, one thread per
  * (chain, parameter), summed per chain in parameter order into L.tlp. */
-__device__ void proposal_priors(const Dev &d, const JobDev &job, const SweepLds &L, int i0, int k) {
+__device__ __forceinline__ void proposal_priors(const Dev &d, const JobDev &job, const SweepLds &L, int i0, int k) {
     const int P = d.P;
     for (int e = threadIdx.x; e < k * P; e += blockDim.x) {
         const int c = e / P, p = e - c * P;
@@ -352,8 +377,8 @@ __device__ void proposal_priors(const Dev &d, const JobDev &job, const SweepLds 
  *  3. per chain: the prior terms summed by 8 lanes; per (chain, condition,
  *     accumulator): the likelihood constants when the chains fit one batch.
  * para >= 0: blocked update of that parameter only. */
-__device__ void proposal_stage1(const Dev &d, const JobDev &job, const SweepLds &L, const Stream &st,
-                                int lo, int hi, int i0, int k, int para) {
+__device__ __forceinline__ void proposal_stage1(const Dev &d, const JobDev &job, const SweepLds &L, const Stream &st,
+                                int lo, int hi, int i0, int k, int para, bool late) {
     const int P = d.P, n = d.nchain;
     const int nb = para >= 0 ? 1 : (P + 3) >> 2;
     for (int e = threadIdx.x; e < k * nb; e += blockDim.x) {
@@ -379,7 +404,9 @@ __device__ void proposal_stage1(const Dev &d, const JobDev &job, const SweepLds 
         }
     }
     const bool hier = job.kind == GGDMC_KIND_HIER;
-    for (int e = threadIdx.x; e < k * P; e += blockDim.x) {
+    /* late: the population step is still running, the priors come after the
+     * likelihoods (late_prior) */
+    for (int e = threadIdx.x; e < (late ? 0 : k * P); e += blockDim.x) {
         const int c = e / P, p = e - c * P;
         double *q = L.pri + (size_t)e * 4;
         if (hier) {
@@ -423,8 +450,8 @@ __device__ __forceinline__ double staged_prior_term(int dist, const double *q, d
     return __builtin_nan("");
 }
 
-__device__ void proposal_stage2(const Dev &d, const JobDev &job, const SweepLds &L, int i0, int k,
-                                int para) {
+__device__ __forceinline__ void proposal_stage2(const Dev &d, const JobDev &job, const SweepLds &L, int i0, int k,
+                                int para, bool late) {
     const int P = d.P;
     for (int e = threadIdx.x; e < k * P; e += blockDim.x) {
         const int c = e / P, p = e - c * P;
@@ -433,12 +460,15 @@ __device__ void proposal_stage2(const Dev &d, const JobDev &job, const SweepLds 
         if (para < 0 || p == para)
             x = xi + (L.jit[e] + job.gamma * (L.th[(size_t)L.m0[c] * P + p] - L.th[(size_t)L.m1[c] * P + p]));
         L.prop[e] = x;
-        L.tm[e] = staged_prior_term(L.pdist[p], L.pri + (size_t)e * 4, L.bnd[2 * p], L.bnd[2 * p + 1], x);
+        /* late: only the support (the prior's other factors come later and
+         * can only turn a finite term into NaN, which rejects as well) */
+        L.tm[e] = late ? (x != x ? x : (x < L.bnd[2 * p] || x > L.bnd[2 * p + 1] ? -__builtin_inf() : 0.0))
+                       : staged_prior_term(L.pdist[p], L.pri + (size_t)e * 4, L.bnd[2 * p], L.bnd[2 * p + 1], x);
     }
 }
 
 /* Returns whether the likelihood constants of all k chains were built. */
-__device__ bool proposal_stage3(const Dev &d, const SweepLds &L, int k, int chain_batch) {
+__device__ __forceinline__ bool proposal_stage3(const Dev &d, const SweepLds &L, int k, int chain_batch) {
     const int P = d.P;
     for (int e = threadIdx.x; e < ((k + 7) & ~7) * 8; e += blockDim.x) {
         const int c = e >> 3, l = e & 7;
@@ -458,7 +488,7 @@ __device__ bool proposal_stage3(const Dev &d, const SweepLds &L, int k, int chai
  * d.csum[(cbase + j) * n + i0 + c]; otherwise their sequential sum goes to
  * L.tll[c]. */
 template <int MODE>
-__device__ void proposals_loglike(const Dev &d, const SweepLds &L, const TrialView &tv, int k,
+__device__ __forceinline__ void proposals_loglike(const Dev &d, const SweepLds &L, const TrialView &tv, int k,
                                   int chain_batch, int cmax, int64_t t0, int64_t t_end, int j_lo,
                                   int nj, bool to_global, int64_t cbase, int i0, bool prebuilt = false) {
     const int P = d.P, n = d.nchain;
@@ -493,7 +523,7 @@ __device__ void proposals_loglike(const Dev &d, const SweepLds &L, const TrialVi
 
 /* Metropolis accept of proposals c < k for chains i0 + c (src/de.cpp:81-108):
  * state updated in LDS and in global memory. */
-__device__ void accept_proposals(const Dev &d, const SweepLds &L, int x, int i0, int k) {
+__device__ __forceinline__ void accept_proposals(const Dev &d, const SweepLds &L, int x, int i0, int k) {
     const int n = d.nchain, P = d.P;
     for (int c = threadIdx.x; c < k; c += blockDim.x) {
         const int i = i0 + c;
@@ -521,7 +551,7 @@ __device__ void accept_proposals(const Dev &d, const SweepLds &L, int x, int i0,
 /* Migration ring of subject x over all its trials (src/de.cpp:615-665,
  * :157-199), in one workgroup; state updated in LDS and global memory. */
 template <int MODE>
-__device__ void subject_migration(const Dev &d, const SweepLds &L, const JobDev &job, int x,
+__device__ __forceinline__ void subject_migration(const Dev &d, const SweepLds &L, const JobDev &job, int x,
                                   const Stream &st, int para, int cmax, const TrialView &tv) {
     const int n = d.nchain, P = d.P;
     if (threadIdx.x == 0) L.flag[0] = subchains(st, n, L.sub);
@@ -578,7 +608,7 @@ __device__ void subject_migration(const Dev &d, const SweepLds &L, const JobDev 
 }
 
 /* Thinned store of subject x's state (src: LDS copy or global state). */
-__device__ void store_subject(const Dev &d, const double *th, const double *lp, const double *ll,
+__device__ __forceinline__ void store_subject(const Dev &d, const double *th, const double *lp, const double *ll,
                               int x, uint32_t iter) {
     if (iter % (uint32_t)d.thin != 0) return;
     const int k = (int)(iter / (uint32_t)d.thin);
@@ -589,115 +619,6 @@ __device__ void store_subject(const Dev &d, const double *th, const double *lp, 
     for (int c = threadIdx.x; c < n; c += blockDim.x) {
         d.out_lp[((size_t)x * d.nmc + k) * n + c] = lp[c];
         d.out_ll[((size_t)x * d.nmc + k) * n + c] = ll[c];
-    }
-}
-
-/* ------------------------------------------------------------ sweep kernel */
-template <int MODE>
-__global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs a) {
-    extern __shared__ __attribute__((aligned(16))) char lds_raw[];
-    const int4 item = a.items[blockIdx.x];
-    const int x = item.x;
-    const int c0 = item.y >> 16, c1 = item.y & 0xFFFF;
-    const int t = item.z >> 16, ntile = item.z & 0xFFFF;
-    const int n = d.nchain, P = d.P, G = d.G;
-    const SweepLds L = carve(lds_raw, n, P, a.kmax, d.n_cond * d.n_acc, a.chain_batch, a.cmax, a.tstage);
-    load_exp2_table(d);
-    /* loads first (they depend on the item only), then the draws */
-    stage_copy(L.th, d.theta + (size_t)x * n * P, n * P);
-    stage_copy(L.lp, d.lp + (size_t)x * n, n);
-    stage_copy(L.ll, d.ll + (size_t)x * n, n);
-    /* the item's trials; staged in LDS when they fit */
-    const int64_t s0 = d.subj_off[x], s1 = d.subj_off[x + 1];
-    const int64_t t_lo = ntile > 1 ? s0 + (int64_t)t * d.tpt : s0;
-    const int64_t t_hi = ntile > 1 ? min(s1, t_lo + (int64_t)d.tpt) : s1;
-    TrialView tv{d.rt, d.tcode, 0};
-    if (a.tstage > 0) {
-        stage_copy(L.trt, d.rt + t_lo, (int)(t_hi - t_lo));
-        stage_copy(L.tcd, d.tcode + t_lo, (int)(t_hi - t_lo));
-        tv = TrialView{L.trt, L.tcd, t_lo};
-    }
-    const JobDev job = d.subj_jobs[x];
-    const uint32_t iter = *d.iter_base + a.iter_off;
-    const bool single = job.kind != GGDMC_KIND_HIER;
-    const bool blocked = d.blocked_sub != 0;
-    const uint64_t seed = ((uint64_t)job.k1 << 32) | job.k0;
-    const uint32_t unit = (uint32_t)d.subj_unit[x];
-
-    /* level decision (src/de.cpp:208-236 single level, :333-372 subjects) */
-    const int dsweep = (blocked && single) ? 0 : a.sweep;
-    const Stream sd = make_stream(seed, iter + d.philox_offset, unit, (uint32_t)dsweep);
-    const bool migrate = decision_u(sd) < d.sub_mig;
-    const int para = blocked ? a.sweep : -1;
-    const Stream st = make_stream(seed, iter + d.philox_offset, unit, (uint32_t)a.sweep);
-    auto group_k = [&](int g, int &lo, int &hi, int &i0) {
-        lo = group_lo(g, n, G);
-        hi = group_lo(g + 1, n, G);
-        i0 = lo + c0;
-        return min(c1 == kAllChains ? hi - lo : c1, hi - lo) - c0;
-    };
-    if (!migrate) {
-        /* stage 1 of the first group overlaps the state loads */
-        int lo, hi, i0;
-        const int k = group_k(a.g_begin, lo, hi, i0);
-        if (k > 0) proposal_stage1(d, job, L, st, lo, hi, i0, k, para);
-    }
-    __syncthreads();
-    /* deferred store: the first launch of iteration i stores iteration i - 1 */
-    if (a.store_prev && c0 == 0 && t == 0 && iter > 1) store_subject(d, L.th, L.lp, L.ll, x, iter - 1);
-
-    if (migrate) {
-        const bool act = blocked && single ? a.sweep == 0 : true;
-        if (c0 == 0 && t == 0 && act && a.g_begin == 0) {
-            const int para = (blocked && !single) ? a.sweep : -1;
-            subject_migration<MODE>(d, L, job, x, sd, para, a.cmax,
-                                    ntile == 1 && a.tstage > 0 ? tv : TrialView{d.rt, d.tcode, 0});
-        }
-    } else {
-        const int j_lo = (int)((t_lo - s0) / kChunk);
-        const int nj = (int)((t_hi - t_lo + kChunk - 1) / kChunk);
-        for (int g = a.g_begin; g < a.g_end; ++g) {
-            int lo, hi, i0;
-            const int k = group_k(g, lo, hi, i0);
-            if (k <= 0) continue;
-            if (g != a.g_begin) {
-                proposal_stage1(d, job, L, st, lo, hi, i0, k, para);
-                __syncthreads();
-            }
-            proposal_stage2(d, job, L, i0, k, para);
-            __syncthreads();
-            const bool built = proposal_stage3(d, L, k, a.chain_batch);
-            __syncthreads();
-            proposals_loglike<MODE>(d, L, tv, k, a.chain_batch, a.cmax, s0, s1, j_lo, nj, ntile > 1,
-                                    d.chunk_off[x], i0, built);
-            if (threadIdx.x == 0) {
-                int na = 0;
-                for (int c = 0; c < k; ++c) na += L.active[c];
-                atomicAdd(d.evals + x, (unsigned long long)na * (unsigned long long)(t_hi - t_lo));
-            }
-            if (ntile > 1) {
-                /* the last-arriving tile of (subject, chain range) accepts */
-                __threadfence();
-                __syncthreads();
-                uint32_t *ctr = d.tile_ctr + (blockIdx.x - t);
-                if (threadIdx.x == 0) {
-                    const uint32_t old = atomicAdd(ctr, 1u);
-                    L.flag[3] = old == (uint32_t)(ntile - 1);
-                }
-                __syncthreads();
-                if (!L.flag[3]) continue;
-                __threadfence();
-                if (threadIdx.x == 0) *ctr = 0u;
-                const int64_t cb = d.chunk_off[x], ncb = d.chunk_off[x + 1] - cb;
-                for (int c = threadIdx.x; c < k; c += blockDim.x) {
-                    double s = 0.0;
-                    for (int64_t j = 0; j < ncb; ++j) s += d.csum[(size_t)(cb + j) * n + i0 + c];
-                    L.tll[c] = s;
-                }
-                __syncthreads();
-            }
-            accept_proposals(d, L, x, i0, k);
-        }
     }
 }
 
@@ -712,19 +633,31 @@ struct PhiLds {
     int *m0, *m1, *acc;      /* [ngmax] */
     int *sub;                /* [n] */
     int *flag;               /* [4] */
+    double *st;              /* [n*P*4] statistics (mean, M2, min, max), or null: in d.stats */
+    double *hc;              /* [2P*6] population prior: p0, p1, lower, upper, cached logs */
+    double *sb;              /* [P*2] bounds of the subject prior */
+    int *hd;                 /* [2P] population prior distribution */
 };
 
-__host__ __device__ inline size_t phi_lds_bytes(int n, int P, int G) {
+__host__ __device__ inline size_t phi_lds_bytes(int n, int P, int G, bool stats_lds) {
     const size_t ng = (size_t)(n + G - 1) / G;
     const size_t dbl = (size_t)n * 2 * P + 2 * (size_t)n + ng * 2 * P + (ng + 1) * 4 * P +
-                       (size_t)n * P * 2 + (ng + 1) * P * 2 + ng;
-    return dbl * sizeof(double) + (3 * ng + n + 4) * sizeof(int);
+                       (size_t)n * P * 2 + (ng + 1) * P * 2 + ng + (stats_lds ? (size_t)n * P * 4 : 0) +
+                       (size_t)P * 12 + (size_t)P * 2;
+    return dbl * sizeof(double) + (3 * ng + n + 4 + 2 * (size_t)P) * sizeof(int);
 }
 
-__device__ PhiLds phi_carve(char *base, int n, int P, int G) {
+__device__ __forceinline__ PhiLds phi_carve(char *base, int n, int P, int G, bool stats_lds) {
     const int ng = (n + G - 1) / G;
     PhiLds L;
     double *p = reinterpret_cast<double *>(base);
+    L.st = nullptr;
+    if (stats_lds) {
+        L.st = p;
+        p += (size_t)n * P * 4;
+    }
+    L.hc = p; p += (size_t)P * 12;
+    L.sb = p; p += (size_t)P * 2;
     L.ph = p; p += (size_t)n * 2 * P;
     L.plp = p; p += n;
     L.pll = p; p += n;
@@ -738,8 +671,69 @@ __device__ PhiLds phi_carve(char *base, int n, int P, int G) {
     L.m1 = q; q += ng;
     L.acc = q; q += ng;
     L.sub = q; q += n;
-    L.flag = q;
+    L.flag = q; q += 4;
+    L.hd = q;
     return L;
+}
+
+/* Stages population state f, its cached prior logarithms (used for chain c
+ * only when d.pcache_valid[f*n + c], those flags in L.sub) and the run's
+ * prior constants in LDS; nothing here depends on the subject statistics, so
+ * these loads overlap theirs. */
+__device__ __forceinline__ void phi_stage(const Dev &d, int f, const JobDev &job, const PhiLds &L) {
+    const int n = d.nchain, P = d.P, P2 = 2 * P;
+    /* every load of the first pass in flight together, then the stores */
+    Piece<double, 2> pph{L.ph, d.phi + (size_t)f * n * P2, n * P2};
+    Piece<double, 2> ppc{L.pc, d.pcache + (size_t)f * n * P2, n * P2};   /* n * P * 2 == n * P2 */
+    Piece<double, 1> plp{L.plp, d.phi_lp + (size_t)f * n, n};
+    Piece<double, 1> pll{L.pll, d.phi_ll + (size_t)f * n, n};
+    Piece<int32_t, 1> pva{L.sub, d.pcache_valid + (size_t)f * n, n};
+    const int q = min((int)threadIdx.x, P2 - 1), p = min((int)threadIdx.x, P - 1);
+    const int k = job.hprior_off + q, kp = job.prior_off + p;
+    pph.load();
+    ppc.load();
+    plp.load();
+    pll.load();
+    pva.load();
+    const double h0 = d.pr_p0[k], h1 = d.pr_p1[k], h2 = d.pr_lo[k], h3 = d.pr_up[k];
+    const double h4 = d.scache[2 * k], h5 = d.scache[2 * k + 1];
+    const int hd = d.pr_dist[k];
+    const double b0 = d.pr_lo[kp], b1 = d.pr_up[kp];
+    pph.store();
+    ppc.store();
+    plp.store();
+    pll.store();
+    pva.store();
+    if ((int)threadIdx.x < P2) {
+        double *h = L.hc + (size_t)q * 6;
+        h[0] = h0;
+        h[1] = h1;
+        h[2] = h2;
+        h[3] = h3;
+        h[4] = h4;
+        h[5] = h5;
+        L.hd[q] = hd;
+    }
+    if ((int)threadIdx.x < P) {
+        L.sb[2 * p] = b0;
+        L.sb[2 * p + 1] = b1;
+    }
+    /* population designs wider than the workgroup (2P > blockDim) */
+    for (int r = threadIdx.x + blockDim.x; r < P2; r += blockDim.x) {
+        const int kk = job.hprior_off + r;
+        double *h = L.hc + (size_t)r * 6;
+        h[0] = d.pr_p0[kk];
+        h[1] = d.pr_p1[kk];
+        h[2] = d.pr_lo[kk];
+        h[3] = d.pr_up[kk];
+        h[4] = d.scache[2 * kk];
+        h[5] = d.scache[2 * kk + 1];
+        L.hd[r] = d.pr_dist[kk];
+        if (r < P) {
+            L.sb[2 * r] = d.pr_lo[job.prior_off + r];
+            L.sb[2 * r + 1] = d.pr_up[job.prior_off + r];
+        }
+    }
 }
 
 /* One population parameter's share of sumloghlike from its sufficient
@@ -749,11 +743,11 @@ __device__ PhiLds phi_carve(char *base, int n, int P, int G) {
  * NaN for sg <= 0 or NaN inputs, -inf when a subject value is outside
  * [lo, up]; summing the P shares in parameter order gives sumloghlike
  * (src/de.cpp:245-270). */
-__device__ double hll_term(const Dev &d, const JobDev &job, int p, double mu, double sg,
-                           const double *lc, const double *st, int S) {
+__device__ __forceinline__ double hll_term(const PhiLds &L, int p, double mu, double sg, const double *lc,
+                           const double *st, int S) {
     const double mean = st[0], M2 = st[1], mn = st[2], mx = st[3];
     if (!(sg > 0.0) || mean != mean || mu != mu) return __builtin_nan("");
-    const double lo = d.pr_lo[job.prior_off + p], up = d.pr_up[job.prior_off + p];
+    const double lo = L.sb[2 * p], up = L.sb[2 * p + 1];
     if (mn < lo || mx > up) return -__builtin_inf();
     const double dm = mean - mu;
     return -(M2 + (double)S * dm * dm) / (2.0 * sg * sg) -
@@ -762,9 +756,9 @@ __device__ double hll_term(const Dev &d, const JobDev &job, int p, double mu, do
 
 /* lc = (log sg, log Z) of parameter p's truncated normal with location mu
  * and scale sg on the subject prior's bounds (Z its mass). */
-__device__ __forceinline__ void phi_log_consts(const Dev &d, const JobDev &job, int p, double mu,
-                                               double sg, double *lc) {
-    const double lo = d.pr_lo[job.prior_off + p], up = d.pr_up[job.prior_off + p];
+__device__ __forceinline__ void phi_log_consts(const PhiLds &L, int p, double mu, double sg,
+                                               double *lc) {
+    const double lo = L.sb[2 * p], up = L.sb[2 * p + 1];
     lc[0] = ::log(sg);
     lc[1] = ::log(tnorm_mass((lo - mu) / sg, (up - mu) / sg));
 }
@@ -774,20 +768,20 @@ __device__ __forceinline__ double hyper_prior_term(const Dev &d, const JobDev &j
     return log_prior1(d.pr_dist[k], d.pr_p0[k], d.pr_p1[k], d.pr_lo[k], d.pr_up[k], x);
 }
 
-/* The same from the session's cached logarithms of the fixed prior. */
-__device__ __forceinline__ double hyper_prior_term_cached(const Dev &d, const JobDev &job, int q, double x) {
+/* The same from the prior's constants and cached logarithms staged in LDS. */
+__device__ __forceinline__ double hyper_prior_term_cached(const PhiLds &L, int q, double x) {
     if (x != x) return x;
-    const int k = job.hprior_off + q;
-    const double p0 = d.pr_p0[k], p1 = d.pr_p1[k];
-    if (d.pr_dist[k] == GGDMC_DIST_UNIF) {
-        if (x < p0 || x > p1) return -__builtin_inf();
-        return -d.scache[2 * k];
+    const double *h = L.hc + (size_t)q * 6;
+    const int dist = L.hd[q];
+    if (dist == GGDMC_DIST_UNIF) {
+        if (x < h[0] || x > h[1]) return -__builtin_inf();
+        return -h[4];
     }
-    if (d.pr_dist[k] == GGDMC_DIST_TNORM) {
-        if (!(p1 > 0.0)) return __builtin_nan("");
-        if (x < d.pr_lo[k] || x > d.pr_up[k]) return -__builtin_inf();
-        const double z = (x - p0) / p1;
-        return -0.5 * z * z - 0.918938533204672741780329736406 - d.scache[2 * k] - d.scache[2 * k + 1];
+    if (dist == GGDMC_DIST_TNORM) {
+        if (!(h[1] > 0.0)) return __builtin_nan("");
+        if (x < h[2] || x > h[3]) return -__builtin_inf();
+        const double z = (x - h[0]) / h[1];
+        return -0.5 * z * z - 0.918938533204672741780329736406 - h[4] - h[5];
     }
     return __builtin_nan("");
 }
@@ -809,37 +803,52 @@ __device__ double hyper_logprior(const Dev &d, const JobDev &job, const double *
  *   sum = sum_j sum_j,  mean = sum / S,  M2 = sum_j (M2_j + n_j (mean_j - mean)^2)
  * (one block: the two-pass M2). Step 1: block statistics of items
  * e = q * nblk + j in [e0, e1), one thread each, into bst[e][4]. */
-__device__ void block_stats(const Dev &d, const JobDev &job, int e0, int e1, double *bst) {
+/* Loads the 1 <= nb <= kStatBlock values src[i * stride] into registers
+ * (clamped indices: the loads issue back to back). */
+__device__ __forceinline__ void block_load(const double *src, size_t stride, int nb, double (&v)[kStatBlock]) {
+#pragma unroll
+    for (int i = 0; i < kStatBlock; ++i) v[i] = src[(size_t)min(i, nb - 1) * stride];
+}
+
+/* Sum, M2 about the mean sum / nb, min, max and a NaN flag of v[0..nb), in
+ * order. */
+__device__ __forceinline__ void block_moments(const double (&v)[kStatBlock], int nb, double &sum,
+                                              double &m2, double &mn, double &mx, bool &nan) {
+    sum = 0.0;
+    mn = __builtin_inf();
+    mx = -__builtin_inf();
+    nan = false;
+#pragma unroll
+    for (int i = 0; i < kStatBlock; ++i) {
+        if (i < nb) {
+            sum += v[i];
+            mn = fmin(mn, v[i]);
+            mx = fmax(mx, v[i]);
+            nan |= v[i] != v[i];
+        }
+    }
+    const double mean = sum / (double)nb;
+    m2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < kStatBlock; ++i) {
+        if (i < nb) {
+            const double dx = v[i] - mean;
+            m2 += dx * dx;
+        }
+    }
+}
+
+__device__ __forceinline__ void block_stats(const Dev &d, const JobDev &job, int e0, int e1, double *bst) {
     const int P = d.P, n = d.nchain, S = job.nsubject;
     const int nblk = (S + kStatBlock - 1) / kStatBlock;
     const size_t stride = (size_t)n * P;
     for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x) {
         const int q = e / nblk, j = e - q * nblk;
         const int s0 = j * kStatBlock, nb = min(kStatBlock, S - s0);
-        const double *src = d.theta + (size_t)(job.first_subject + s0) * stride + q;
-        double v[kStatBlock];
-#pragma unroll
-        for (int i = 0; i < kStatBlock; ++i) v[i] = i < nb ? src[i * stride] : 0.0;
-        double sum = 0.0, mn = __builtin_inf(), mx = -__builtin_inf();
-        bool nan = false;
-#pragma unroll
-        for (int i = 0; i < kStatBlock; ++i) {
-            if (i < nb) {
-                sum += v[i];
-                mn = fmin(mn, v[i]);
-                mx = fmax(mx, v[i]);
-                nan |= v[i] != v[i];
-            }
-        }
-        const double mean = sum / (double)nb;
-        double m2 = 0.0;
-#pragma unroll
-        for (int i = 0; i < kStatBlock; ++i) {
-            if (i < nb) {
-                const double dx = v[i] - mean;
-                m2 += dx * dx;
-            }
-        }
+        double v[kStatBlock], sum, m2, mn, mx;
+        bool nan;
+        block_load(d.theta + (size_t)(job.first_subject + s0) * stride + q, stride, nb, v);
+        block_moments(v, nb, sum, m2, mn, mx, nan);
         double *o = bst + (size_t)e * 4;
         o[0] = nan ? __builtin_nan("") : sum;
         o[1] = m2;
@@ -848,9 +857,22 @@ __device__ void block_stats(const Dev &d, const JobDev &job, int e0, int e1, dou
     }
 }
 
+/* A run of at most kStatBlock subjects: block_stats and combine_stats in one
+ * step, from values already loaded, straight into o[4] (the same values:
+ * with one block the combination's correction term is nb * 0^2). */
+__device__ __forceinline__ void direct_stats(const double (&v)[kStatBlock], int S, double *o) {
+    double sum, m2, mn, mx;
+    bool nan;
+    block_moments(v, S, sum, m2, mn, mx, nan);
+    o[0] = nan ? __builtin_nan("") : sum / (double)S;
+    o[1] = m2;
+    o[2] = mn;
+    o[3] = mx;
+}
+
 /* Step 2: combine the blocks of pairs [q0, q1) in order into out[q][4] =
  * (mean, M2, min, max). */
-__device__ void combine_stats(const JobDev &job, int q0, int q1, const double *bst, double *out) {
+__device__ __forceinline__ void combine_stats(const JobDev &job, int q0, int q1, const double *bst, double *out) {
     const int S = job.nsubject;
     const int nblk = (S + kStatBlock - 1) / kStatBlock;
     for (int q = q0 + (int)threadIdx.x; q < q1; q += blockDim.x) {
@@ -880,7 +902,7 @@ __device__ void combine_stats(const JobDev &job, int q0, int q1, const double *b
  * proposal, its prior and likelihood terms, and the refreshed likelihoods of
  * source and target (hierarchical runs, src/de.cpp:494-500) in parallel, the
  * sums and the accept by one thread. */
-__device__ void phi_migration(const Dev &d, const JobDev &job, const PhiLds &L, const Stream &sd,
+__device__ __forceinline__ void phi_migration(const Dev &d, const JobDev &job, const PhiLds &L, const Stream &sd,
                               int para, const double *stats, bool hier) {
     const int n = d.nchain, P = d.P, P2 = 2 * P, S = job.nsubject;
     if (threadIdx.x == 0) L.flag[0] = subchains(sd, n, L.sub);
@@ -912,17 +934,17 @@ __device__ void phi_migration(const Dev &d, const JobDev &job, const PhiLds &L, 
         for (int e = threadIdx.x; e < nterm; e += blockDim.x) {
             double v;
             if (e < P2) {
-                v = hyper_prior_term_cached(d, job, e, L.prop[e]);
+                v = hyper_prior_term_cached(L, e, L.prop[e]);
             } else if (e < 3 * P) {
                 const int p = e - P2;
                 double *lc = L.pp + (size_t)p * 2;
-                phi_log_consts(d, job, p, L.prop[p], L.prop[P + p], lc);
-                v = hll_term(d, job, p, L.prop[p], L.prop[P + p], lc, st_src + p * 4, S);
+                phi_log_consts(L, p, L.prop[p], L.prop[P + p], lc);
+                v = hll_term(L, p, L.prop[p], L.prop[P + p], lc, st_src + p * 4, S);
             } else {
                 const int which = e < 4 * P ? src : dst;
                 const int p = e - (e < 4 * P ? 3 * P : 4 * P);
                 const double *ph = L.ph + (size_t)which * P2;
-                v = hll_term(d, job, p, ph[p], ph[P + p], L.pc + ((size_t)which * P + p) * 2,
+                v = hll_term(L, p, ph[p], ph[P + p], L.pc + ((size_t)which * P + p) * 2,
                              stats + ((size_t)which * P + p) * 4, S);
             }
             L.tm[e] = v;
@@ -963,7 +985,7 @@ __device__ void phi_migration(const Dev &d, const JobDev &job, const PhiLds &L, 
  * partners per chain, jitter per (chain, Philox block), prior terms per
  * (chain, parameter of 2P), likelihood shares per (chain, parameter) for the
  * proposal and the refreshed current state (src/de.cpp:397), sums per chain. */
-__device__ void phi_crossover(const Dev &d, const JobDev &job, const PhiLds &L, const Stream &st,
+__device__ __forceinline__ void phi_crossover(const Dev &d, const JobDev &job, const PhiLds &L, const Stream &st,
                               int lo, int hi, int para, const double *stats, bool hier) {
     const int n = d.nchain, P = d.P, P2 = 2 * P, S = job.nsubject, ng = hi - lo;
     /* partners, accept U and jitter: one thread per (chain, Philox block) */
@@ -996,23 +1018,23 @@ __device__ void phi_crossover(const Dev &d, const JobDev &job, const PhiLds &L, 
      * that a wave runs one kind of term */
     for (int e = threadIdx.x; e < ng * P2; e += blockDim.x) {
         const int c = e / P2, q = e - c * P2;
-        L.tm[(size_t)c * 4 * P + q] = hyper_prior_term_cached(d, job, q, L.prop[e]);
+        L.tm[(size_t)c * 4 * P + q] = hyper_prior_term_cached(L, q, L.prop[e]);
     }
     for (int e = threadIdx.x; e < ng * P; e += blockDim.x) {
         const int c = e / P, p = e - c * P;
         const int i = lo + c;
         const double *x = L.prop + (size_t)c * P2;
         double *lc = L.pp + (size_t)e * 2;
-        phi_log_consts(d, job, p, x[p], x[P + p], lc);
+        phi_log_consts(L, p, x[p], x[P + p], lc);
         L.tm[(size_t)c * 4 * P + P2 + p] =
-            hll_term(d, job, p, x[p], x[P + p], lc, stats + ((size_t)(hier ? i : 0) * P + p) * 4, S);
+            hll_term(L, p, x[p], x[P + p], lc, stats + ((size_t)(hier ? i : 0) * P + p) * 4, S);
     }
     if (hier) {
         for (int e = threadIdx.x; e < ng * P; e += blockDim.x) {
             const int c = e / P, p = e - c * P;
             const int i = lo + c;
             const double *cur = L.ph + (size_t)i * P2;
-            L.tm[(size_t)c * 4 * P + 3 * P + p] = hll_term(d, job, p, cur[p], cur[P + p],
+            L.tm[(size_t)c * 4 * P + 3 * P + p] = hll_term(L, p, cur[p], cur[P + p],
                                                            L.pc + ((size_t)i * P + p) * 2,
                                                            stats + ((size_t)i * P + p) * 4, S);
         }
@@ -1049,30 +1071,19 @@ __device__ void phi_crossover(const Dev &d, const JobDev &job, const PhiLds &L, 
     __syncthreads();
 }
 
-/* Population step of state f for iteration iter in one workgroup, from the
- * per-chain statistics in d.stats (src/de.cpp:272-383 decisions). */
-__device__ void phi_update(const Dev &d, int f, const JobDev &job, uint32_t iter, const PhiLds &L) {
+/* Population step of state f for iteration iter in one workgroup, on the
+ * state staged by phi_stage and the per-chain statistics `stats` (LDS or
+ * global; src/de.cpp:272-383 decisions). */
+__device__ __forceinline__ void phi_update(const Dev &d, int f, const JobDev &job, uint32_t iter, const PhiLds &L,
+                           const double *stats) {
     const int n = d.nchain, P = d.P, P2 = 2 * P, G = d.G;
     const bool hier = job.kind == GGDMC_KIND_HIER;
-    const double *stats = d.stats + (size_t)f * n * P * 4;
-    for (int e = threadIdx.x; e < n * P2; e += blockDim.x) L.ph[e] = d.phi[(size_t)f * n * P2 + e];
-    for (int c = threadIdx.x; c < n; c += blockDim.x) {
-        L.plp[c] = d.phi_lp[(size_t)f * n + c];
-        L.pll[c] = d.phi_ll[(size_t)f * n + c];
-    }
-    __syncthreads();
     /* log scale / log mass of every chain's subject prior: kept from the last
-     * population step, computed afresh after a start or restore */
-    const bool have = d.pcache_valid[f] != 0;
+     * population step (staged), computed afresh after a start or restore */
     for (int e = threadIdx.x; e < n * P; e += blockDim.x) {
         const int c = e / P, p = e - c * P;
-        if (have) {
-            L.pc[2 * e] = d.pcache[((size_t)f * n * P + e) * 2];
-            L.pc[2 * e + 1] = d.pcache[((size_t)f * n * P + e) * 2 + 1];
-        } else {
-            phi_log_consts(d, job, p, L.ph[(size_t)c * P2 + p], L.ph[(size_t)c * P2 + P + p],
-                           L.pc + (size_t)e * 2);
-        }
+        if (L.sub[c] == 0)
+            phi_log_consts(L, p, L.ph[(size_t)c * P2 + p], L.ph[(size_t)c * P2 + P + p], L.pc + (size_t)e * 2);
     }
     __syncthreads();
     const uint32_t unit = hier ? kUnitPhi : 0u;
@@ -1103,7 +1114,7 @@ __device__ void phi_update(const Dev &d, int f, const JobDev &job, uint32_t iter
         d.phi_lp[(size_t)f * n + c] = L.plp[c];
         d.phi_ll[(size_t)f * n + c] = L.pll[c];
     }
-    if (threadIdx.x == 0) d.pcache_valid[f] = 1;
+    for (int c = threadIdx.x; c < n; c += blockDim.x) d.pcache_valid[(size_t)f * n + c] = 1;
     if (iter % (uint32_t)d.thin == 0) {
         const int k = (int)(iter / (uint32_t)d.thin);
         if (k < d.nmc) {
@@ -1120,25 +1131,47 @@ __device__ void phi_update(const Dev &d, int f, const JobDev &job, uint32_t iter
 /* grid = n_phi * nwg: workgroup (f, b) computes the block statistics of items
  * [b * blockDim, (b + 1) * blockDim) of a hierarchical run; the last
  * workgroup of each run combines them and performs the population step.
+ * With one workgroup per run it stages the population state while the
+ * subject values load, and a run of at most kStatBlock subjects needs no
+ * block pass. Statistics go to LDS when stats_lds (else to d.stats).
  * Hyper-only runs have fixed statistics: their workgroup b == 0 steps
  * directly. */
-__global__ __launch_bounds__(kPhiBlock) void phi_kernel(Dev d, uint32_t iter_off, int nwg) {
-    extern __shared__ __attribute__((aligned(16))) char lds_raw[];
+__device__ __forceinline__ void phi_block(const Dev &d, uint32_t iter, int nwg, int stats_lds, int bid,
+                                          char *lds_raw, bool publish) {
     const int n = d.nchain, P = d.P;
-    const int f = blockIdx.x / nwg, b = blockIdx.x - f * nwg;
+    const int f = bid / nwg, b = bid - f * nwg;
     const JobDev job = d.phi_jobs[f];
-    const uint32_t iter = *d.iter_base + iter_off;
-    const PhiLds L = phi_carve(lds_raw, n, P, d.G);
+    const PhiLds L = phi_carve(lds_raw, n, P, d.G, stats_lds != 0);
+    double *gst = d.stats + (size_t)f * n * P * 4;
+    double *st = L.st ? L.st : gst;
+    const int nblk = (job.nsubject + kStatBlock - 1) / kStatBlock;
+    const int nitem = n * P * nblk;
+    double *bst = d.bstats + (size_t)f * d.bstats_stride;
     if (job.kind != GGDMC_KIND_HIER) {
         if (b != 0) return;
-        phi_update(d, f, job, iter, L);
-        return;
-    }
-    const int nitem = n * P * ((job.nsubject + kStatBlock - 1) / kStatBlock);
-    double *bst = d.bstats + (size_t)f * d.bstats_stride;
-    if (b * (int)blockDim.x < nitem)
-        block_stats(d, job, b * (int)blockDim.x, min(nitem, (b + 1) * (int)blockDim.x), bst);
-    if (nwg > 1) {
+        phi_stage(d, f, job, L);
+        if (L.st) stage_copy(L.st, gst, P * 4);
+    } else if (nwg == 1 && nblk == 1) {
+        /* this thread's subject values load together with the staging */
+        const int nq = n * P, S = job.nsubject;
+        const double *base = d.theta + (size_t)job.first_subject * nq;
+        const int q0 = min((int)threadIdx.x, nq - 1);
+        double v[kStatBlock];
+        block_load(base + q0, (size_t)nq, S, v);
+        phi_stage(d, f, job, L);
+        if ((int)threadIdx.x < nq) direct_stats(v, S, st + (size_t)q0 * 4);
+        for (int q = threadIdx.x + blockDim.x; q < nq; q += blockDim.x) {
+            block_load(base + q, (size_t)nq, S, v);
+            direct_stats(v, S, st + (size_t)q * 4);
+        }
+    } else if (nwg == 1) {
+        phi_stage(d, f, job, L);
+        block_stats(d, job, 0, nitem, bst);
+        __syncthreads();
+        combine_stats(job, 0, n * P, bst, st);
+    } else {
+        if (b * (int)blockDim.x < nitem)
+            block_stats(d, job, b * (int)blockDim.x, min(nitem, (b + 1) * (int)blockDim.x), bst);
         /* the last workgroup of the run continues */
         __threadfence();
         __syncthreads();
@@ -1150,12 +1183,431 @@ __global__ __launch_bounds__(kPhiBlock) void phi_kernel(Dev d, uint32_t iter_off
         if (!L.flag[0]) return;
         __threadfence();
         if (threadIdx.x == 0) d.stats_ctr[f] = 0u;
-    } else {
-        __syncthreads();
+        phi_stage(d, f, job, L);
+        combine_stats(job, 0, n * P, bst, st);
     }
-    combine_stats(job, 0, n * P, bst, d.stats + (size_t)f * n * P * 4);
     __syncthreads();
-    phi_update(d, f, job, iter, L);
+    phi_update(d, f, job, iter, L, st);
+    if (publish) {
+        /* release: every thread's stores of the state, then the flag */
+        __threadfence();
+        __syncthreads();
+        for (int c = threadIdx.x; c < n; c += blockDim.x)
+            __hip_atomic_store(d.phi_ready + (size_t)f * n + c, iter, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__global__ __launch_bounds__(kPhiBlock) void phi_kernel(Dev d, uint32_t iter_off, int nwg, int stats_lds) {
+    extern __shared__ __attribute__((aligned(16))) char lds_raw[];
+    phi_block(d, *d.iter_base + iter_off, nwg, stats_lds, (int)blockIdx.x, lds_raw, false);
+}
+
+/* Whether iteration iter of population state job needs the whole-run
+ * population step (phi_block) rather than one workgroup per chain: hyper
+ * runs, blocked population sweeps, and iterations that migrate
+ * (src/de.cpp:272-383 decisions, the draws phi_update makes). */
+__device__ __forceinline__ bool phi_full(const Dev &d, const JobDev &job, uint32_t iter) {
+    if (job.kind != GGDMC_KIND_HIER || d.blocked_pop != 0) return true;
+    const uint64_t seed = ((uint64_t)job.k1 << 32) | job.k0;
+    return decision_u(make_stream(seed, iter + d.philox_offset, kUnitPhi, 0u)) < d.pop_mig;
+}
+
+__host__ __device__ inline size_t phi_chain_lds_bytes(int P, int nblk) {
+    const size_t dbl = (size_t)P * (6 + 2 + 4 + 2 + 2 + 4 + 12 + 2) + 4 + (nblk > 1 ? (size_t)P * nblk * 4 : 0);
+    return dbl * sizeof(double) + ((size_t)2 * P + 4) * sizeof(int);
+}
+
+/* Chain i's crossover of population state f in its own workgroup, for an
+ * iteration that is a plain crossover of a hierarchical run (phi_full false):
+ * the statistics of chain i's subject values, the proposal, its prior and
+ * likelihood terms, the refreshed current likelihood and the accept: the
+ * operations of block_stats / combine_stats and phi_crossover (the same
+ * values), for one chain. Chain i's partners come from the other chain
+ * groups, whose states do not change in this launch. Publishes
+ * d.phi_ready[f*n + i] = iter after the state is stored. */
+__device__ __forceinline__ void phi_chain(const Dev &d, int f, int i, uint32_t iter, char *lds_raw) {
+    const int n = d.nchain, P = d.P, P2 = 2 * P, G = d.G;
+    const JobDev job = d.phi_jobs[f];
+    const int S = job.nsubject, nblk = (S + kStatBlock - 1) / kStatBlock;
+    const int t = threadIdx.x, bs = blockDim.x;
+    int g = 0;
+    while (group_lo(g + 1, n, G) <= i) ++g;
+    const uint64_t seed = ((uint64_t)job.k1 << 32) | job.k0;
+    const Stream st = make_stream(seed, iter + d.philox_offset, kUnitPhi, 0u);
+    int m0, m1;
+    double U;
+    partners(st, i, group_lo(g, n, G), group_lo(g + 1, n, G), n, m0, m1, U);
+
+    PhiLds L{};   /* the term helpers read hc, hd, sb */
+    double *p = reinterpret_cast<double *>(lds_raw);
+    double *xi = p; p += P2;      /* rows i, m0, m1 */
+    double *x0 = p; p += P2;
+    double *x1 = p; p += P2;
+    double *prop = p; p += P2;
+    double *tm = p; p += 4 * P;
+    double *pc = p; p += 2 * P;
+    double *pp = p; p += 2 * P;
+    double *stt = p; p += 4 * P;
+    L.hc = p; p += 12 * P;
+    L.sb = p; p += 2 * P;
+    double *misc = p; p += 4;     /* lp, ll after the step */
+    double *bst = p; p += nblk > 1 ? (size_t)P * nblk * 4 : 0;
+    int *q = reinterpret_cast<int *>(p);
+    L.hd = q; q += P2;
+    int *flag = q;                /* [0] pcache valid, [1] accepted */
+
+    const size_t off = (size_t)f * n;
+    const size_t nq = (size_t)n * P;
+    const double *tbase = d.theta + (size_t)job.first_subject * nq + (size_t)i * P;
+    /* loads in one batch: the subject values of this chain (one block),
+     * rows i, m0, m1, chain i's cached logarithms, the prior constants */
+    double v[kStatBlock];
+    if (nblk == 1) block_load(tbase + min(t, P - 1), nq, S, v);
+    {
+        Piece<double, 1> pxi{xi, d.phi + (off + i) * P2, P2};
+        Piece<double, 1> px0{x0, d.phi + (off + m0) * P2, P2};
+        Piece<double, 1> px1{x1, d.phi + (off + m1) * P2, P2};
+        Piece<double, 1> ppc{pc, d.pcache + (off + i) * P2, P2};
+        const int qq = min(t, P2 - 1), pq = min(t, P - 1);
+        const int k = job.hprior_off + qq, kp = job.prior_off + pq;
+        pxi.load();
+        px0.load();
+        px1.load();
+        ppc.load();
+        const double h0 = d.pr_p0[k], h1 = d.pr_p1[k], h2 = d.pr_lo[k], h3 = d.pr_up[k];
+        const double h4 = d.scache[2 * k], h5 = d.scache[2 * k + 1];
+        const int hd = d.pr_dist[k];
+        const double b0 = d.pr_lo[kp], b1 = d.pr_up[kp];
+        const double lp0 = d.phi_lp[off + i];
+        const int valid = d.pcache_valid[off + i];
+        pxi.store();
+        px0.store();
+        px1.store();
+        ppc.store();
+        if (t < P2) {
+            double *h = L.hc + (size_t)qq * 6;
+            h[0] = h0;
+            h[1] = h1;
+            h[2] = h2;
+            h[3] = h3;
+            h[4] = h4;
+            h[5] = h5;
+            L.hd[qq] = hd;
+        }
+        if (t < P) {
+            L.sb[2 * pq] = b0;
+            L.sb[2 * pq + 1] = b1;
+        }
+        if (t == 0) {
+            misc[0] = lp0;
+            flag[0] = valid;
+        }
+        for (int r = t + bs; r < P2; r += bs) {
+            const int kk = job.hprior_off + r;
+            double *h = L.hc + (size_t)r * 6;
+            h[0] = d.pr_p0[kk];
+            h[1] = d.pr_p1[kk];
+            h[2] = d.pr_lo[kk];
+            h[3] = d.pr_up[kk];
+            h[4] = d.scache[2 * kk];
+            h[5] = d.scache[2 * kk + 1];
+            L.hd[r] = d.pr_dist[kk];
+            if (r < P) {
+                L.sb[2 * r] = d.pr_lo[job.prior_off + r];
+                L.sb[2 * r + 1] = d.pr_up[job.prior_off + r];
+            }
+        }
+    }
+    /* statistics of chain i's values of each parameter */
+    if (nblk == 1) {
+        if (t < P) direct_stats(v, S, stt + (size_t)t * 4);
+        for (int r = t + bs; r < P; r += bs) {
+            block_load(tbase + r, nq, S, v);
+            direct_stats(v, S, stt + (size_t)r * 4);
+        }
+    } else {
+        for (int e = t; e < P * nblk; e += bs) {
+            const int pr = e / nblk, j = e - pr * nblk;
+            const int nb = min(kStatBlock, S - j * kStatBlock);
+            double sum, m2, mn, mx;
+            bool nan;
+            block_load(tbase + (size_t)j * kStatBlock * nq + pr, nq, nb, v);
+            block_moments(v, nb, sum, m2, mn, mx, nan);
+            double *o = bst + (size_t)e * 4;
+            o[0] = nan ? __builtin_nan("") : sum;
+            o[1] = m2;
+            o[2] = mn;
+            o[3] = mx;
+        }
+    }
+    __syncthreads();
+    if (nblk > 1) combine_stats(job, 0, P, bst, stt);
+    /* chain i's cached logarithms after a start or restore */
+    if (flag[0] == 0)
+        for (int r = t; r < P; r += bs) phi_log_consts(L, r, xi[r], xi[P + r], pc + (size_t)r * 2);
+    /* the proposal: jitter of parameter e is word e & 3 of Philox block 1 + e / 4 */
+    for (int e = t; e < P2; e += bs) {
+        const U4 r = st.block((uint32_t)i, 1u + (uint32_t)(e >> 2));
+        const int j = e & 3;   /* selects, not an indexed (private-memory) array */
+        const uint32_t wj = j == 0 ? r.x[0] : (j == 1 ? r.x[1] : (j == 2 ? r.x[2] : r.x[3]));
+        prop[e] = xi[e] + (jitter_word(wj, d.rp) + job.gamma * (x0[e] - x1[e]));
+    }
+    __syncthreads();
+    /* [0, 2P) prior, [2P, 3P) proposal shares, [3P, 4P) refreshed current shares */
+    for (int e = t; e < 4 * P; e += bs) {
+        double val;
+        if (e < P2) {
+            val = hyper_prior_term_cached(L, e, prop[e]);
+        } else if (e < 3 * P) {
+            const int r = e - P2;
+            phi_log_consts(L, r, prop[r], prop[P + r], pp + (size_t)r * 2);
+            val = hll_term(L, r, prop[r], prop[P + r], pp + (size_t)r * 2, stt + (size_t)r * 4, S);
+        } else {
+            const int r = e - 3 * P;
+            val = hll_term(L, r, xi[r], xi[P + r], pc + (size_t)r * 2, stt + (size_t)r * 4, S);
+        }
+        tm[e] = val;
+    }
+    __syncthreads();
+    if (t < 8) {
+        const double tlp = group8_sum(tm, P2, t);
+        const double tll = group8_sum(tm + P2, P, t);
+        const double ref = group8_sum(tm + 3 * P, P, t);
+        if (t == 0) {
+            const double cur = ref + misc[0];
+            const bool a = mh_accept(tlp + tll, cur, U);
+            flag[1] = a;
+            misc[2] = a ? tlp : misc[0];
+            misc[3] = a ? tll : ref;
+        }
+    }
+    __syncthreads();
+    const bool acc = flag[1] != 0;
+    const bool out = iter % (uint32_t)d.thin == 0 && (int)(iter / (uint32_t)d.thin) < d.nmc;
+    const size_t ko = out ? (size_t)f * d.nmc + iter / (uint32_t)d.thin : 0;
+    for (int e = t; e < P2; e += bs) {
+        const double x = acc ? prop[e] : xi[e];
+        d.phi[(off + i) * P2 + e] = x;
+        d.pcache[(off + i) * P2 + e] = acc ? pp[e] : pc[e];   /* P * 2 == P2 entries per chain */
+        if (out) d.phi_out_theta[(ko * n + i) * P2 + e] = x;
+    }
+    if (t == 0) {
+        d.phi_lp[off + i] = misc[2];
+        d.phi_ll[off + i] = misc[3];
+        d.pcache_valid[off + i] = 1;
+        if (out) {
+            d.phi_out_lp[ko * n + i] = misc[2];
+            d.phi_out_ll[ko * n + i] = misc[3];
+        }
+    }
+    /* release: every thread's stores, then the flag */
+    __threadfence();
+    __syncthreads();
+    if (t == 0) __hip_atomic_store(d.phi_ready + off + i, iter, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/* A subject workgroup that needs population chains [c_lo, c_lo + c_n) of
+ * state f for iteration iter: threads poll d.phi_ready with device-scope
+ * atomic loads a bounded number of times (on giving up they raise d.fault,
+ * which the host reports as an error). The writer released the state before
+ * the flag; the reader then either reads it with device-scope loads (no
+ * fence: late_prior) or, with acquire, fences once and reads it plainly (a
+ * fence writes back and invalidates the whole L2 of the XCD, costly when
+ * hundreds of workgroups do it). */
+constexpr uint32_t kSpinLimit = 1u << 18;
+
+__device__ __forceinline__ void wait_chains(const Dev &d, int f, int c_lo, int c_n, uint32_t iter,
+                                            bool acquire) {
+    for (int c = threadIdx.x; c < c_n; c += blockDim.x) {
+        const uint32_t *flag = d.phi_ready + (size_t)f * d.nchain + c_lo + c;
+        uint32_t spins = 0;
+        while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != iter) {
+            /* after one failure every later wait gives up at once */
+            if (++spins == kSpinLimit ||
+                __hip_atomic_load(d.fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                atomicOr(d.fault, 1u);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(4);
+        }
+    }
+    __syncthreads();
+    if (acquire) __threadfence();
+}
+
+/* Prior terms of proposals L.prop[0..k) for chains i0 + c under the
+ * population state just published (read with device-scope loads): the
+ * operations of proposal_stage1/2 (location, scale and cached logarithms per
+ * (chain, parameter)), summed per chain in parameter order by 8 lanes into
+ * L.tlp, as proposal_stage3. */
+__device__ __forceinline__ double load_agent(const double *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void late_prior(const Dev &d, const JobDev &job, const SweepLds &L, int i0, int k) {
+    const int P = d.P, n = d.nchain;
+    for (int e = threadIdx.x; e < k * P; e += blockDim.x) {
+        const int c = e / P, p = e - c * P;
+        const size_t r = ((size_t)job.phi_index * n + i0 + c) * P + p;
+        const double *ph = d.phi + ((size_t)job.phi_index * n + i0 + c) * (2 * P);
+        const double q[4] = {load_agent(ph + p), load_agent(ph + P + p), load_agent(d.pcache + 2 * r),
+                             load_agent(d.pcache + 2 * r + 1)};
+        L.tm[e] = staged_prior_term(L.pdist[p], q, L.bnd[2 * p], L.bnd[2 * p + 1], L.prop[e]);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < ((k + 7) & ~7) * 8; e += blockDim.x) {
+        const int c = e >> 3, l = e & 7;
+        const double v = group8_sum(L.tm + (size_t)(c < k ? c : 0) * P, P, l);
+        if (c < k && l == 0) L.tlp[c] = v;
+    }
+    __syncthreads();
+}
+
+/* ------------------------------------------------------------ sweep kernel */
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs a) {
+    extern __shared__ __attribute__((aligned(16))) char lds_raw[];
+    if ((int)blockIdx.x < a.phi_wgs) {
+        /* the population step of the launch's chains, concurrent with the
+         * subjects' proposals and likelihoods (they depend on it only through
+         * the priors of the hierarchical subjects' proposals): one workgroup
+         * per (state, chain); whole-run steps by the first chain's workgroup
+         * of the first group */
+        const int c_lo = group_lo(a.g_begin, d.nchain, d.G);
+        const int nc = group_lo(a.g_end, d.nchain, d.G) - c_lo;
+        const int f = (int)blockIdx.x / nc, i = c_lo + (int)blockIdx.x % nc;
+        const uint32_t it = *d.iter_base + a.iter_off;
+        if (!phi_full(d, d.phi_jobs[f], it)) phi_chain(d, f, i, it, lds_raw);
+        else if (a.g_begin == 0 && i == 0) phi_block(d, it, 1, a.phi_stats_lds, f, lds_raw, true);
+        return;
+    }
+    const int ib = (int)blockIdx.x - a.phi_wgs;
+    const int4 item = a.items[ib];
+    const int x = item.x;
+    const int c0 = item.y >> 16, c1 = item.y & 0xFFFF;
+    const int t = item.z >> 16, ntile = item.z & 0xFFFF;
+    const int n = d.nchain, P = d.P, G = d.G;
+    const SweepLds L = carve(lds_raw, n, P, a.kmax, d.n_cond * d.n_acc, a.chain_batch, a.cmax, a.tstage);
+    /* the item's trials; staged in LDS when they fit */
+    const int64_t s0 = d.subj_off[x], s1 = d.subj_off[x + 1];
+    const int64_t t_lo = ntile > 1 ? s0 + (int64_t)t * d.tpt : s0;
+    const int64_t t_hi = ntile > 1 ? min(s1, t_lo + (int64_t)d.tpt) : s1;
+    const int nst = a.tstage > 0 ? (int)(t_hi - t_lo) : 0;
+    {
+        /* loads first (they depend on the item only), all in flight together,
+         * then the draws */
+        Piece<double, 1> pex{g_exp2tab, d.exp2tab, 64};
+        Piece<double, 4> pth{L.th, d.theta + (size_t)x * n * P, n * P};
+        Piece<double, 1> plp{L.lp, d.lp + (size_t)x * n, n};
+        Piece<double, 1> pll{L.ll, d.ll + (size_t)x * n, n};
+        Piece<double, 4> prt{L.trt, d.rt + t_lo, nst};
+        Piece<uint16_t, 4> pcd{L.tcd, d.tcode + t_lo, nst};
+        pex.load();
+        pth.load();
+        plp.load();
+        pll.load();
+        prt.load();
+        pcd.load();
+        pex.store();
+        pth.store();
+        plp.store();
+        pll.store();
+        prt.store();
+        pcd.store();
+    }
+    const TrialView tv = nst > 0 ? TrialView{L.trt, L.tcd, t_lo} : TrialView{d.rt, d.tcode, 0};
+    const JobDev job = d.subj_jobs[x];
+    const uint32_t iter = *d.iter_base + a.iter_off;
+    const bool single = job.kind != GGDMC_KIND_HIER;
+    const bool blocked = d.blocked_sub != 0;
+    const bool late = a.phi_wgs > 0 && !single;
+    const uint64_t seed = ((uint64_t)job.k1 << 32) | job.k0;
+    const uint32_t unit = (uint32_t)d.subj_unit[x];
+
+    /* level decision (src/de.cpp:208-236 single level, :333-372 subjects) */
+    const int dsweep = (blocked && single) ? 0 : a.sweep;
+    const Stream sd = make_stream(seed, iter + d.philox_offset, unit, (uint32_t)dsweep);
+    const bool migrate = decision_u(sd) < d.sub_mig;
+    const int para = blocked ? a.sweep : -1;
+    const Stream st = make_stream(seed, iter + d.philox_offset, unit, (uint32_t)a.sweep);
+    auto group_k = [&](int g, int &lo, int &hi, int &i0) {
+        lo = group_lo(g, n, G);
+        hi = group_lo(g + 1, n, G);
+        i0 = lo + c0;
+        return min(c1 == kAllChains ? hi - lo : c1, hi - lo) - c0;
+    };
+    if (!migrate) {
+        /* stage 1 of the first group overlaps the state loads */
+        int lo, hi, i0;
+        const int k = group_k(a.g_begin, lo, hi, i0);
+        if (k > 0) proposal_stage1(d, job, L, st, lo, hi, i0, k, para, late);
+    }
+    __syncthreads();
+    /* deferred store: the first launch of iteration i stores iteration i - 1 */
+    if (a.store_prev && c0 == 0 && t == 0 && iter > 1) store_subject(d, L.th, L.lp, L.ll, x, iter - 1);
+
+    if (migrate) {
+        const bool act = blocked && single ? a.sweep == 0 : true;
+        /* with the population step in these launches, a hierarchical
+         * subject migrates in the last group's launch, after every chain's
+         * population step */
+        const bool here = late ? a.g_end == G : a.g_begin == 0;
+        if (c0 == 0 && t == 0 && act && here) {
+            const int para = (blocked && !single) ? a.sweep : -1;
+            if (late) wait_chains(d, job.phi_index, 0, n, iter, true);
+            subject_migration<MODE>(d, L, job, x, sd, para, a.cmax,
+                                    ntile == 1 && a.tstage > 0 ? tv : TrialView{d.rt, d.tcode, 0});
+        }
+    } else {
+        const int j_lo = (int)((t_lo - s0) / kChunk);
+        const int nj = (int)((t_hi - t_lo + kChunk - 1) / kChunk);
+        for (int g = a.g_begin; g < a.g_end; ++g) {
+            int lo, hi, i0;
+            const int k = group_k(g, lo, hi, i0);
+            if (k <= 0) continue;
+            if (g != a.g_begin) {
+                proposal_stage1(d, job, L, st, lo, hi, i0, k, para, late);
+                __syncthreads();
+            }
+            proposal_stage2(d, job, L, i0, k, para, late);
+            __syncthreads();
+            const bool built = proposal_stage3(d, L, k, a.chain_batch);
+            __syncthreads();
+            proposals_loglike<MODE>(d, L, tv, k, a.chain_batch, a.cmax, s0, s1, j_lo, nj, ntile > 1,
+                                    d.chunk_off[x], i0, built);
+            if (threadIdx.x == 0) {
+                int na = 0;
+                for (int c = 0; c < k; ++c) na += L.active[c];
+                atomicAdd(d.evals + x, (unsigned long long)na * (unsigned long long)(t_hi - t_lo));
+            }
+            if (ntile > 1) {
+                /* the last-arriving tile of (subject, chain range) accepts */
+                __threadfence();
+                __syncthreads();
+                uint32_t *ctr = d.tile_ctr + (ib - t);
+                if (threadIdx.x == 0) {
+                    const uint32_t old = atomicAdd(ctr, 1u);
+                    L.flag[3] = old == (uint32_t)(ntile - 1);
+                }
+                __syncthreads();
+                if (!L.flag[3]) continue;
+                __threadfence();
+                if (threadIdx.x == 0) *ctr = 0u;
+                const int64_t cb = d.chunk_off[x], ncb = d.chunk_off[x + 1] - cb;
+                for (int c = threadIdx.x; c < k; c += blockDim.x) {
+                    double s = 0.0;
+                    for (int64_t j = 0; j < ncb; ++j) s += d.csum[(size_t)(cb + j) * n + i0 + c];
+                    L.tll[c] = s;
+                }
+                __syncthreads();
+            }
+            if (late) {
+                wait_chains(d, job.phi_index, i0, k, iter, false);
+                late_prior(d, job, L, i0, k);
+            }
+            accept_proposals(d, L, x, i0, k);
+        }
+    }
 }
 
 /* ------------------------------------------------------- plain FP64 path */
@@ -1445,11 +1897,16 @@ size_t ggdmc_internal_sweep_lds(int n, int P, int kmax, int per_chain, int chain
                                 int tstage) {
     return sweep_lds_bytes(n, P, kmax, per_chain, chain_batch, cmax, tstage);
 }
-size_t ggdmc_internal_phi_lds(int n, int P, int G) { return phi_lds_bytes(n, P, G); }
+size_t ggdmc_internal_phi_lds(int n, int P, int G, int stats_lds) {
+    return phi_lds_bytes(n, P, G, stats_lds != 0);
+}
+size_t ggdmc_internal_phi_chain_lds(int P, int max_subjects) {
+    return phi_chain_lds_bytes(P, (max_subjects + kStatBlock - 1) / kStatBlock);
+}
 
 hipError_t ggdmc_internal_launch_sweep(const Dev *d, const SweepArgs *a, int mode, size_t lds,
                                        hipStream_t stream) {
-    const dim3 grid(a->n_items), block(kBlock);
+    const dim3 grid(a->phi_wgs + a->n_items), block(kBlock);
     if (mode == GGDMC_MIXED_BATCHED)
         subject_sweep_kernel<MODE_MIXED><<<grid, block, lds, stream>>>(*d, *a);
     else
@@ -1457,11 +1914,12 @@ hipError_t ggdmc_internal_launch_sweep(const Dev *d, const SweepArgs *a, int mod
     return hipGetLastError();
 }
 
-hipError_t ggdmc_internal_launch_phi(const Dev *d, uint32_t iter_off, size_t lds, hipStream_t stream) {
+hipError_t ggdmc_internal_launch_phi(const Dev *d, uint32_t iter_off, size_t lds, int stats_lds,
+                                     hipStream_t stream) {
     /* items of the largest run: pairs x subject blocks */
     const int nwg = (int)((d->bstats_stride / 4 + kPhiBlock - 1) / kPhiBlock);
-    phi_kernel<<<dim3(d->n_phi * std::max(nwg, 1)), dim3(kPhiBlock), lds, stream>>>(*d, iter_off,
-                                                                                   std::max(nwg, 1));
+    phi_kernel<<<dim3(d->n_phi * std::max(nwg, 1)), dim3(kPhiBlock), lds, stream>>>(
+        *d, iter_off, std::max(nwg, 1), stats_lds);
     return hipGetLastError();
 }
 
