@@ -252,6 +252,8 @@ struct SweepLds {
     int *sub;               /* [n] migration subset */
     int *pdist;             /* [P] prior distribution of each parameter */
     int *flag;              /* [4] */
+    double *pcst;           /* [per_chain*6] design constants (par_const) */
+    int32_t *psrc;          /* [per_chain*6] design sources (par_src) */
 };
 
 __host__ __device__ inline size_t sweep_lds_bytes(int n, int P, int kmax, int per_chain,
@@ -259,8 +261,8 @@ __host__ __device__ inline size_t sweep_lds_bytes(int n, int P, int kmax, int pe
     const size_t dbl = (size_t)n * P + 2 * (size_t)n + 7 * (size_t)kmax * P + 2 * (size_t)P +
                        3 * (size_t)kmax + (size_t)chain_batch * per_chain * kAccKDoubles +
                        (size_t)kmax * cmax;
-    const size_t ints = 4 * (size_t)kmax + n + P + 4;
-    return (dbl + (size_t)tstage) * sizeof(double) + ints * sizeof(int) +
+    const size_t ints = 4 * (size_t)kmax + n + P + 4 + (size_t)per_chain * 6;
+    return (dbl + (size_t)tstage + (size_t)per_chain * 6) * sizeof(double) + ints * sizeof(int) +
            (((size_t)tstage * sizeof(uint16_t) + 7) & ~(size_t)7);
 }
 
@@ -282,6 +284,7 @@ __device__ __forceinline__ SweepLds carve(char *base, int n, int P, int kmax, in
     L.acck = p; p += (size_t)chain_batch * per_chain * kAccKDoubles;
     L.csum = p; p += (size_t)kmax * cmax;
     L.trt = p; p += tstage;
+    L.pcst = p; p += (size_t)per_chain * 6;
     L.tcd = reinterpret_cast<uint16_t *>(p);
     p += ((size_t)tstage * sizeof(uint16_t) + 7) / 8;
     int *q = reinterpret_cast<int *>(p);
@@ -291,7 +294,8 @@ __device__ __forceinline__ SweepLds carve(char *base, int n, int P, int kmax, in
     L.acc = q; q += kmax;
     L.sub = q; q += n;
     L.pdist = q; q += P;
-    L.flag = q;
+    L.flag = q; q += 4;
+    L.psrc = q;
     return L;
 }
 
@@ -341,12 +345,13 @@ struct Piece {
 };
 
 /* Fills the accumulator constants of proposals [cb, cb+kb). */
-__device__ __forceinline__ void build_acck(const Dev &d, const double *prop, int P, int cb, int kb, double *acck) {
+__device__ __forceinline__ void build_acck(const Dev &d, const SweepLds &L, const double *prop, int P, int cb, int kb,
+                                           double *acck) {
     const int per_chain = d.n_cond * d.n_acc;
     AccK *K = reinterpret_cast<AccK *>(acck);
     for (int e = threadIdx.x; e < kb * per_chain; e += blockDim.x) {
         const int c = e / per_chain, r = e - c * per_chain;
-        make_acck<DevOps>(prop + (size_t)(cb + c) * P, d.par_src + r * 6, d.par_const + r * 6, K[e]);
+        make_acck<DevOps>(prop + (size_t)(cb + c) * P, L.psrc + r * 6, L.pcst + r * 6, K[e]);
     }
 }
 
@@ -424,11 +429,6 @@ __device__ __forceinline__ void proposal_stage1(const Dev &d, const JobDev &job,
             q[3] = d.scache[2 * kk + 1];
         }
     }
-    for (int p = threadIdx.x; p < P; p += blockDim.x) {
-        L.bnd[2 * p] = d.pr_lo[job.prior_off + p];
-        L.bnd[2 * p + 1] = d.pr_up[job.prior_off + p];
-        L.pdist[p] = hier ? GGDMC_DIST_TNORM : d.pr_dist[job.prior_off + p];
-    }
 }
 
 /* Prior term of x from the staged constants q = (location or lower end,
@@ -479,7 +479,7 @@ __device__ __forceinline__ bool proposal_stage3(const Dev &d, const SweepLds &L,
         }
     }
     const bool all = k <= chain_batch;
-    if (all) build_acck(d, L.prop, P, 0, k, L.acck);
+    if (all) build_acck(d, L, L.prop, P, 0, k, L.acck);
     return all;
 }
 
@@ -497,7 +497,7 @@ __device__ __forceinline__ void proposals_loglike(const Dev &d, const SweepLds &
         const int kb = min(chain_batch, k - cb);
         if (!(prebuilt && cb == 0)) {
             __syncthreads();
-            build_acck(d, L.prop, P, cb, kb, L.acck);
+            build_acck(d, L, L.prop, P, cb, kb, L.acck);
         }
         __syncthreads();
         for (int jb = 0; jb < nj; jb += cmax) {
@@ -1493,10 +1493,18 @@ __global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs 
     const int64_t t_lo = ntile > 1 ? s0 + (int64_t)t * d.tpt : s0;
     const int64_t t_hi = ntile > 1 ? min(s1, t_lo + (int64_t)d.tpt) : s1;
     const int nst = a.tstage > 0 ? (int)(t_hi - t_lo) : 0;
+    const JobDev job = d.subj_jobs[x];
     {
         /* loads first (they depend on the item only), all in flight together,
-         * then the draws */
+         * then the draws: exp table, state, trials, design rows, the subject
+         * prior's bounds and families */
+        const int per = d.n_cond * d.n_acc * 6;
         Piece<double, 1> pex{g_exp2tab, d.exp2tab, 64};
+        Piece<int32_t, 1> psr{L.psrc, d.par_src, per};
+        Piece<double, 1> pcs{L.pcst, d.par_const, per};
+        const int tp = (int)threadIdx.x, pq = min(tp, P - 1);
+        const double blo = d.pr_lo[job.prior_off + pq], bup = d.pr_up[job.prior_off + pq];
+        const int pdi = job.kind == GGDMC_KIND_HIER ? GGDMC_DIST_TNORM : d.pr_dist[job.prior_off + pq];
         Piece<double, 4> pth{L.th, d.theta + (size_t)x * n * P, n * P};
         Piece<double, 1> plp{L.lp, d.lp + (size_t)x * n, n};
         Piece<double, 1> pll{L.ll, d.ll + (size_t)x * n, n};
@@ -1508,15 +1516,28 @@ __global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs 
         pll.load();
         prt.load();
         pcd.load();
+        psr.load();
+        pcs.load();
         pex.store();
         pth.store();
         plp.store();
         pll.store();
         prt.store();
         pcd.store();
+        psr.store();
+        pcs.store();
+        if (tp < P) {
+            L.bnd[2 * tp] = blo;
+            L.bnd[2 * tp + 1] = bup;
+            L.pdist[tp] = pdi;
+        }
+        for (int r = tp + (int)blockDim.x; r < P; r += blockDim.x) {
+            L.bnd[2 * r] = d.pr_lo[job.prior_off + r];
+            L.bnd[2 * r + 1] = d.pr_up[job.prior_off + r];
+            L.pdist[r] = job.kind == GGDMC_KIND_HIER ? GGDMC_DIST_TNORM : d.pr_dist[job.prior_off + r];
+        }
     }
     const TrialView tv = nst > 0 ? TrialView{L.trt, L.tcd, t_lo} : TrialView{d.rt, d.tcode, 0};
-    const JobDev job = d.subj_jobs[x];
     const uint32_t iter = *d.iter_base + a.iter_off;
     const bool single = job.kind != GGDMC_KIND_HIER;
     const bool blocked = d.blocked_sub != 0;
@@ -1878,6 +1899,8 @@ __global__ __launch_bounds__(kBlock) void loglike_kernel(Dev d, const double *th
     const int q = blockIdx.x;
     load_exp2_table(d);
     for (int p = threadIdx.x; p < P; p += blockDim.x) L.prop[p] = thetas[(size_t)q * P + p];
+    stage_copy(L.psrc, d.par_src, d.n_cond * d.n_acc * 6);
+    stage_copy(L.pcst, d.par_const, d.n_cond * d.n_acc * 6);
     if (threadIdx.x == 0) L.active[0] = 1;
     __syncthreads();
     const int64_t t0 = d.subj_off[s], t1 = d.subj_off[s + 1];
