@@ -44,11 +44,13 @@ W_ALG = {2: 130.0, 4: 255.0}
 # VALU lane-slots the subject sweep kernel actually issues per evaluation,
 # everything included (precision work, indexing, staging, sums, proposals):
 # rocprofv3 SQ_INSTS_VALU x 64 / evaluations of a launch (profiles/), at the
-# decomposition the autotuner picks there (2 chains x all trials per item).
-W_ISSUED = {"c3": 677.0}
+# decomposition the autotuner picks there (c3: 2 chains x all trials per item,
+# c5: a whole chain group per item); the population workgroups of the launch
+# are included.
+W_ISSUED = {"c3": 703.0, "c5": 900.0}
 # HBM bytes per launch of the subject sweep kernel: rocprofv3 FETCH_SIZE x 2.0
 # + WRITE_SIZE x 1.0 (factors of a 256 MiB copy kernel; profiles/).
-TRAFFIC = {"c3": 4.128e6}
+TRAFFIC = {"c3": 4.746e6}
 # 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
 VALU_PEAK_TSLOTS = 256 * 4 * 16 * 2.4e9 / 1e12
 

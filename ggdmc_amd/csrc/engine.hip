@@ -6,9 +6,11 @@
  *   batched path : per iteration one subject launch per (sweep, chain group)
  *                  over work items (subject, chain range, trial tile), or one
  *                  per sweep with one item per subject over every group; the
- *                  population step runs in its own launch or at the end of
- *                  the last subject launch; blocks of iterations are captured
- *                  in a hipGraph and replayed.
+ *                  population step runs inside the subject launches (one
+ *                  workgroup per chain of the launch's group) or, with one
+ *                  item per subject or blocked subject sweeps, in its own
+ *                  launch first; blocks of iterations are captured in a
+ *                  hipGraph and replayed.
  *   plain path   : the reference's sequential order, one launch per chain
  *                  step, no graph (the FP64 comparator).
  * The reference's own driver loop is de_class::run_chains / run_hchains
