@@ -301,6 +301,38 @@ GGDMC_FN double mixed_trial_term_flagged(const AccK *kc, int n_acc, int w, doubl
     return nan ? nanv : ((zero || neg) ? kLogDblEps : lp + acc);
 }
 
+/* Two independent trials' terms (A and B) in one body: both winner terms,
+ * then each loser step of both, so that the two dependency chains interleave
+ * in a wave that has few neighbours to hide latency with. Per trial the same
+ * operations as mixed_trial_term_flagged, hence the same values. */
+template <class Ops>
+GGDMC_FN void mixed_trial_term_flagged2(const AccK *kA, int wA, double rtA, const AccK *kB, int wB,
+                                        double rtB, int n_acc, double &outA, double &outB, bool &illA,
+                                        bool &illB) {
+    const double lpA = mixed_winner_logpdf<Ops>(kA[wA], rtA, illA);
+    const double lpB = mixed_winner_logpdf<Ops>(kB[wB], rtB, illB);
+    double accA = 0.0, accB = 0.0, nanvA = lpA, nanvB = lpB;
+    int negA = 0, negB = 0;
+    bool zeroA = false, zeroB = false, nanA = lpA != lpA, nanB = lpB != lpB;
+    for (int l = 0; l + 1 < n_acc; ++l) {
+        const int jA = l + (l >= wA), jB = l + (l >= wB);
+        const double SA = mixed_loser_survivor<Ops>(kA[jA], rtA, illA);
+        const double SB = mixed_loser_survivor<Ops>(kB[jB], rtB, illB);
+        const bool snA = SA != SA, snB = SB != SB;
+        nanA |= snA;
+        nanB |= snB;
+        nanvA = snA ? SA : nanvA;
+        nanvB = snB ? SB : nanvB;
+        const double aSA = std::fabs(SA), aSB = std::fabs(SB);
+        fold_survivor(SA, mixed_log<Ops>(aSA > 0.0 ? aSA : 1.0), accA, negA, zeroA);
+        fold_survivor(SB, mixed_log<Ops>(aSB > 0.0 ? aSB : 1.0), accB, negB, zeroB);
+    }
+    illA = illA && !nanA;
+    illB = illB && !nanB;
+    outA = nanA ? nanvA : ((zeroA || negA) ? kLogDblEps : lpA + accA);
+    outB = nanB ? nanvB : ((zeroB || negB) ? kLogDblEps : lpB + accB);
+}
+
 /* Batched-path trial term: mixed evaluation, or the reference formula when
  * any accumulator's evaluation is ill-conditioned. */
 template <class Ops>

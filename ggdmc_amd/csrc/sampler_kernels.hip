@@ -133,8 +133,9 @@ __device__ __forceinline__ void chunk_sums(const Dev &d, const TrialView &tv, in
             const AccK *k1 = kc + (code1 >> 3) * d.n_acc, *k2 = kc + (code2 >> 3) * d.n_acc;
             if (MODE == MODE_MIXED) {
                 bool ill1 = false, ill2 = false;
-                const double ma = mixed_trial_term_flagged<DevOps>(k1, d.n_acc, code1 & 7, rt1, ill1);
-                const double mb = mixed_trial_term_flagged<DevOps>(k2, d.n_acc, code2 & 7, rt2, ill2);
+                double ma, mb;
+                mixed_trial_term_flagged2<DevOps>(k1, code1 & 7, rt1, k2, code2 & 7, rt2, d.n_acc, ma, mb, ill1,
+                                                  ill2);
                 a = h1 ? ma : 0.0;
                 b = h2 ? mb : 0.0;
                 /* rare: ill-conditioned trials take the reference formula */
