@@ -68,8 +68,9 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--profile-iters", type=int, default=20)
     ap.add_argument("--decomp", default="",
-                    help="fixed work items 'k,t' (chains per item or -1, trials per item or 0) "
-                         "instead of autotuning")
+                    help="fixed work items 'k,t[,m]' (chains per item or -1, trials per item or 0, "
+                         "launch mode 1 = a subject launch per chain group, 2 = one for every "
+                         "group) instead of autotuning")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device-to-device copy (counter calibration)")
     return ap.parse_args()
@@ -149,17 +150,20 @@ def main():
     thin = max(1, total // 1000)
     nmc = total // thin + 2
     if a.decomp:
-        k, t = (int(v) for v in a.decomp.split(","))
-        cfg = w.config(nmc=nmc, thin=thin, chains_per_item=k, trials_per_tile=t)
+        dv = [int(v) for v in a.decomp.split(",")]
+        k, t, m = dv[0], dv[1], (dv[2] if len(dv) > 2 else 1)
+        cfg = w.config(nmc=nmc, thin=thin, chains_per_item=k, trials_per_tile=t, launch_mode=m)
     else:
         cfg = w.config(nmc=nmc, thin=thin)
     sess = g.Session(cfg, w.model, w.jobs)
-    kci, tpt, nitems = sess.decomposition()
-    print("[bench] decomp %d,%d" % (kci, tpt), file=sys.stderr, flush=True)
+    kci, tpt, nitems, lmode = sess.decomposition()
+    print("[bench] decomp %d,%d,%d" % (kci, tpt, lmode), file=sys.stderr, flush=True)
     print("[bench] workload %s: %s; %d subjects, %d trials; items: %s chains x %s trials, "
-          "%d per launch; setup %.1fs"
+          "%d per launch, %s; setup %.1fs"
           % (w.name, w.description, w.n_subjects, w.n_trials,
-             "all groups" if kci < 0 else kci, tpt or "all", nitems, time.time() - t_build),
+             "all groups" if kci < 0 else kci, tpt or "all", nitems,
+             "one subject launch for every chain group" if lmode == 2 else "a subject launch per chain group",
+             time.time() - t_build),
           file=sys.stderr, flush=True)
 
     sess.advance(a.warmup)
@@ -244,7 +248,8 @@ def main():
         "dtype": "fp32/fp64 mixed (fp64 exp, sums, state)", "data": "synthetic",
         "config": {"workload": "%s: %s" % (w.name, w.description), "subjects": w.n_subjects,
                    "trials": w.n_trials, "chains": w.nchain, "sweep_groups": 2,
-                   "chains_per_item": kci, "trials_per_item": tpt, "items_per_launch": nitems},
+                   "chains_per_item": kci, "trials_per_item": tpt, "items_per_launch": nitems,
+                   "launch_mode": lmode},
         "trial_evals_per_s": round(evals_per_s, 1),
         "plain_fp64_iter_per_s": round(plain, 3) if plain else None,
         "speedup_vs_plain_fp64": round(iters_per_s / plain, 2) if plain else None,

@@ -41,7 +41,7 @@ class Config(C.Structure):
                 ("is_hblocked", C.c_int32), ("is_pblocked", C.c_int32),
                 ("precision", C.c_int32), ("sweep_groups", C.c_int32),
                 ("trials_per_tile", C.c_int32), ("chains_per_item", C.c_int32),
-                ("iter_offset", C.c_uint64)]
+                ("launch_mode", C.c_int32), ("iter_offset", C.c_uint64)]
 
 
 class Model(C.Structure):
@@ -115,7 +115,7 @@ def lib():
     L.ggdmc_session_profile.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double),
                                         C.POINTER(C.c_int32)]
     L.ggdmc_session_trial_evals.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
-    L.ggdmc_session_decomposition.argtypes = [C.c_void_p, _i32, _i32, _i32]
+    L.ggdmc_session_decomposition.argtypes = [C.c_void_p, _i32, _i32, _i32, _i32]
     L.ggdmc_session_destroy.argtypes = [C.c_void_p]
     L.ggdmc_session_destroy.restype = None
     _LIB = L

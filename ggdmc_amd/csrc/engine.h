@@ -88,6 +88,8 @@ struct Dev {
     uint32_t *tile_ctr;         /* [items] tiles of a (subject, chain range) finished */
     uint32_t *phi_ready;        /* [F][n] last iteration whose population step of a chain is stored */
     uint32_t *fault;            /* [1] nonzero: a wait for phi_ready gave up (results invalid) */
+    uint32_t *item_ready;       /* [S][G][rmax] last iteration whose (subject, group, chain range)
+                                   step is stored (all groups in one launch) */
     unsigned long long *evals;  /* [S] trial-likelihood evaluations per subject */
     const uint32_t *iter_base;  /* device iteration base (graph replays) */
 
@@ -95,11 +97,11 @@ struct Dev {
     double *phi_out_theta, *phi_out_lp, *phi_out_ll;
 };
 
-/* A work item of a subject launch: subject x, chains [c0, c1) of the chain
- * group (c1 = kAllChains: to the group's end), trial tile t of ntile, and w =
- * the number of items of subject x in the launch. Packed as int4:
- * {x, c0 << 16 | c1, t << 16 | ntile, w}. The tiles of one (x, c0) are
- * consecutive, in tile order. */
+/* A work item of a subject launch: subject x, chains [c0, c1) of chain group
+ * g (c1 = kAllChains: to the group's end), trial tile t of ntile, and r = the
+ * index of the chain range in the group (c0 / chains per item). Packed as
+ * int4: {x, c0 << 16 | c1, t << 16 | ntile, g << 16 | r}. The tiles of one
+ * (x, g, c0) are consecutive, in tile order. */
 constexpr int kAllChains = 0xFFFF;
 
 /* Per-launch arguments of the subject sweep kernel. */
@@ -117,6 +119,9 @@ struct SweepArgs {
     int32_t phi_wgs;      /* leading workgroups that run the population step of the launch's
                              chains, one per (state, chain) (0: none) */
     int32_t phi_stats_lds; /* a whole-run step keeps the statistics in LDS */
+    int32_t chained;      /* every chain group in this launch: items carry their group, an
+                             item waits for the previous group's items of its subject */
+    int32_t rmax;         /* chain ranges per group (item_ready stride) */
 };
 
 } // namespace ggdmc

@@ -132,6 +132,8 @@ struct ggdmc_session {
      * kBlock threads, one per (state, chain) of the launch's group) instead
      * of its own kernel: */
     bool overlap = false;
+    bool chained = false;         /* every chain group in one subject launch */
+    int rmax = 1;                 /* chain ranges per group (item_ready stride) */
     int ovl_stats_lds = 0;
     size_t ovl_lds = 0;           /* LDS of those launches: max(sweep, population) */
     int nsweep_sub = 1;
@@ -161,7 +163,7 @@ struct ggdmc_session {
     }
 
     void build(const ggdmc_config *c, const ggdmc_model *model, const ggdmc_job *js, int nj);
-    void set_decomp(bool fused_new, int k_new, int tpt_new);
+    void set_decomp(bool fused_new, int k_new, int tpt_new, bool chained_new = false);
     void default_decomp();
     void enqueue_iteration(uint32_t off, std::vector<hipEvent_t> *ev = nullptr);
     void flush_store();
@@ -527,15 +529,18 @@ void ggdmc_session::build(const ggdmc_config *c, const ggdmc_model *model, const
  * fill the chip: chains per item so that subjects x chain ranges >= 1024. */
 void ggdmc_session::default_decomp() {
     const int ngmax = (d.nchain + d.G - 1) / d.G;
+    require(cfg.launch_mode >= 0 && cfg.launch_mode <= 2, "launch_mode must be 0, 1 or 2");
+    const bool one = cfg.launch_mode == 2;
     if (cfg.chains_per_item < 0) {
         require(cfg.trials_per_tile == 0, "one item per subject needs trials_per_tile = 0");
+        require(!one, "one item per subject already covers every group in one launch");
         set_decomp(true, 0, 0);
     } else if (cfg.chains_per_item > 0 || cfg.trials_per_tile > 0) {
-        set_decomp(false, cfg.chains_per_item, cfg.trials_per_tile);
+        set_decomp(false, cfg.chains_per_item, cfg.trials_per_tile, one);
     } else {
         int k = ngmax;
         while (k > 1 && (int64_t)d.S * ((ngmax + k - 1) / k) < 1024) k = (k + 1) / 2;
-        set_decomp(false, k, 0);
+        set_decomp(false, k, 0, one);
     }
 }
 
@@ -543,12 +548,15 @@ void ggdmc_session::default_decomp() {
  * subject's chain ranges of k_new chains x trial tiles of tpt_new trials
  * (rounded up to a multiple of kChunk; 0 = whole subject); or, fused, one
  * item per subject that sweeps every group in one launch. Sizes the LDS. */
-void ggdmc_session::set_decomp(bool fused_new, int k_new, int tpt_new) {
+void ggdmc_session::set_decomp(bool fused_new, int k_new, int tpt_new, bool chained_new) {
     if (tpt_new > 0) tpt_new = ((tpt_new + kChunk - 1) / kChunk) * kChunk;
     require(!fused_new || tpt_new == 0, "one item per subject needs whole subjects");
+    require(!(fused_new && chained_new), "one item per subject already covers every group");
+    require(!chained_new || nsweep_sub == 1, "every group in one launch needs unblocked subject sweeps");
     const int n = d.nchain, G = d.G;
     const int ngmax = (n + G - 1) / G;
     fused = fused_new;
+    chained = chained_new;
     tpt = tpt_new;
     d.tpt = tpt;
     kchain = fused ? ngmax : std::min(k_new > 0 ? k_new : ngmax, ngmax);
@@ -557,6 +565,8 @@ void ggdmc_session::set_decomp(bool fused_new, int k_new, int tpt_new) {
     size_t max_items = 1;
     int64_t max_chunks_item = 1;
     const int nl = fused ? 1 : G;
+    std::vector<int4> all_items;   /* every group's items, in group order (chained) */
+    rmax = 1;
     for (int g = 0; g < nl; ++g) {
         const int ng = fused ? ngmax : (((g + 1) * n) / G - (g * n) / G);
         const int nch = fused ? 1 : (ng + kchain - 1) / kchain;
@@ -565,21 +575,34 @@ void ggdmc_session::set_decomp(bool fused_new, int k_new, int tpt_new) {
             const int64_t ntr = subj_ntr[x];
             const int64_t nt = (tpt == 0 || ntr <= tpt) ? 1 : (ntr + tpt - 1) / tpt;
             require(nt <= 65535, "trials_per_tile too small for a subject's trial count");
-            const int64_t w = (int64_t)nch * nt;
-            require(w < (int64_t)1 << 30, "too many work items per subject");
+            require((int64_t)nch * nt < (int64_t)1 << 30, "too many work items per subject");
             for (int ch = 0; ch < nch; ++ch) {
                 const int c0 = ch * kchain;
                 const int c1 = (fused || ch == nch - 1) ? kAllChains : c0 + kchain;
                 for (int64_t t = 0; t < nt; ++t)
-                    items.push_back(make_int4(x, (c0 << 16) | c1, ((int)t << 16) | (int)nt, (int)w));
+                    items.push_back(make_int4(x, (c0 << 16) | c1, ((int)t << 16) | (int)nt, (g << 16) | ch));
             }
             max_chunks_item = std::max<int64_t>(max_chunks_item,
                                                 nt == 1 ? (ntr + kChunk - 1) / kChunk : tpt / kChunk);
+        }
+        rmax = std::max(rmax, nch);
+        if (chained) {
+            all_items.insert(all_items.end(), items.begin(), items.end());
+            continue;
         }
         require(items.size() < ((size_t)1 << 31), "too many work items");
         list_n.push_back((int)items.size());
         lists.push_back(pool.upload(items, stream));
         max_items = std::max(max_items, items.size());
+    }
+    if (chained) {
+        require(all_items.size() < ((size_t)1 << 31), "too many work items");
+        list_n.push_back((int)all_items.size());
+        lists.push_back(pool.upload(all_items, stream));
+        max_items = std::max(max_items, all_items.size());
+        const size_t nf = (size_t)std::max(d.S, 1) * G * rmax;
+        d.item_ready = pool.alloc<uint32_t>(nf);
+        HIPCHECK(hipMemsetAsync(d.item_ready, 0, sizeof(uint32_t) * nf, stream));
     }
     d.tile_ctr = pool.alloc<uint32_t>(max_items);
     HIPCHECK(hipMemsetAsync(d.tile_ctr, 0, sizeof(uint32_t) * max_items, stream));
@@ -627,15 +650,16 @@ void ggdmc_session::enqueue_iteration(uint32_t off, std::vector<hipEvent_t> *ev)
     if (d.n_phi > 0 && !overlap) HIPCHECK(ggdmc_internal_launch_phi(&d, off, phi_lds, phi_stats_lds, stream));
     if (d.S == 0) return;
     const int lk = mode == GGDMC_FP64_BATCHED ? GGDMC_FP64_BATCHED : GGDMC_MIXED_BATCHED;
-    const int ngl = fused ? 1 : d.G;
+    const int ngl = (fused || chained) ? 1 : d.G;
     for (int sw = 0; sw < nsweep_sub; ++sw) {
         for (int g = 0; g < ngl; ++g) {
             const bool first = sw == 0 && g == 0;
             const bool with_phi = overlap && sw == 0;
-            const int nc = ((g + 1) * d.nchain) / d.G - (g * d.nchain) / d.G;   /* chains of group g */
-            SweepArgs a{off, sw, fused ? 0 : g, fused ? d.G : g + 1, (int)first, chain_batch,
-                        kmax, cmax, tstage, list_n[g], lists[g],
-                        with_phi ? d.n_phi * nc : 0, ovl_stats_lds};
+            /* chains of the launch: group g, or every group */
+            const int nc = chained ? d.nchain : ((g + 1) * d.nchain) / d.G - (g * d.nchain) / d.G;
+            SweepArgs a{off, sw, (fused || chained) ? 0 : g, (fused || chained) ? d.G : g + 1, (int)first,
+                        chain_batch, kmax, cmax, tstage, list_n[g], lists[g],
+                        with_phi ? d.n_phi * nc : 0, ovl_stats_lds, (int)chained, rmax};
             hipEvent_t e0 = nullptr, e1 = nullptr;
             if (ev) {
                 HIPCHECK(hipEventCreate(&e0));
@@ -727,18 +751,22 @@ void ggdmc_session::autotune() {
     struct Cand {
         bool fused;
         int k, tpt;
+        bool chained;
     };
     const int ngmax = (d.nchain + d.G - 1) / d.G;
-    std::vector<Cand> cands{{true, 0, 0}};
+    std::vector<Cand> cands{{true, 0, 0, false}};
     std::vector<int> ks;
     for (int k : {1, 2, 4, 8, ngmax})
         if (k <= ngmax && std::find(ks.begin(), ks.end(), k) == ks.end()) ks.push_back(k);
-    for (int k : ks)
-        for (int t : {0, 256, 1024})
-            if (t == 0 || t < max_ntr) {
-                const int64_t nt = t == 0 ? 1 : (max_ntr + t - 1) / t;
-                if ((int64_t)d.S * ((ngmax + k - 1) / k) * nt <= ((int64_t)1 << 22)) cands.push_back({false, k, t});
-            }
+    const bool can_chain = nsweep_sub == 1 && cfg.launch_mode != 1;
+    for (int one = 0; one < (can_chain ? 2 : 1); ++one)
+        for (int k : ks)
+            for (int t : {0, 256, 1024})
+                if (t == 0 || t < max_ntr) {
+                    const int64_t nt = t == 0 ? 1 : (max_ntr + t - 1) / t;
+                    if ((int64_t)d.S * ((ngmax + k - 1) / k) * nt <= ((int64_t)1 << 22))
+                        cands.push_back({false, k, t, one == 1});
+                }
     const size_t nth = (size_t)d.S * d.nchain * d.P, nch = (size_t)d.S * d.nchain;
     const size_t nph = (size_t)d.n_phi * d.nchain * 2 * d.P, npc = (size_t)d.n_phi * d.nchain;
     const size_t nev = (size_t)std::max(d.S, 1);
@@ -771,6 +799,9 @@ void ggdmc_session::autotune() {
             /* the iterations run again: their population steps are not done */
             HIPCHECK(hipMemsetAsync(d.phi_ready, 0, sizeof(uint32_t) * nf, stream));
         }
+        if (chained)
+            HIPCHECK(hipMemsetAsync(d.item_ready, 0, sizeof(uint32_t) * (size_t)std::max(d.S, 1) * d.G * rmax,
+                                    stream));
         HIPCHECK(hipStreamSynchronize(stream));
     };
     save();
@@ -779,7 +810,7 @@ void ggdmc_session::autotune() {
     Cand pick = cands[0];
     const int K = 4;
     for (const Cand &c : cands) {
-        set_decomp(c.fused, c.k, c.tpt);
+        set_decomp(c.fused, c.k, c.tpt, c.chained);
         next_iter = it0;
         capture(K);
         run_batched(K); /* warm-up */
@@ -797,7 +828,7 @@ void ggdmc_session::autotune() {
         }
     }
     next_iter = it0;
-    set_decomp(pick.fused, pick.k, pick.tpt);
+    set_decomp(pick.fused, pick.k, pick.tpt, pick.chained);
 }
 
 static int fail(const std::exception &e) {
@@ -841,7 +872,8 @@ int ggdmc_session_create(const ggdmc_config *cfg, const ggdmc_model *model, cons
         s->build(cfg, model, jobs, njobs);
         if (s->mode != GGDMC_FP64_PLAIN) {
             /* tune the tile shape when the run is long enough to repay it */
-            if (cfg->trials_per_tile == 0 && cfg->chains_per_item == 0 && s->last_iter >= 64)
+            if (cfg->trials_per_tile == 0 && cfg->chains_per_item == 0 && cfg->launch_mode != 2 &&
+                s->last_iter >= 64)
                 s->autotune();
             const uint32_t total = s->last_iter;
             const int K = (int)std::min<uint32_t>(64u, total);
@@ -883,8 +915,10 @@ int ggdmc_session_elapsed_ms(ggdmc_session *s, float *ms) {
 }
 
 int ggdmc_session_decomposition(ggdmc_session *s, int32_t *chains_per_item,
-                                int32_t *trials_per_tile, int32_t *items_per_launch) {
+                                int32_t *trials_per_tile, int32_t *items_per_launch,
+                                int32_t *launch_mode) {
     if (!s) return GGDMC_EINVAL;
+    if (launch_mode) *launch_mode = s->chained ? 2 : 1;
     if (chains_per_item) *chains_per_item = s->fused ? -1 : s->kchain;
     if (trials_per_tile) *trials_per_tile = s->tpt;
     if (items_per_launch)

@@ -539,12 +539,14 @@ __device__ __forceinline__ void accept_proposals(const Dev &d, const SweepLds &L
         }
     }
     __syncthreads();
+    /* device-scope stores: a later group's item of this subject may read the
+     * rows in the same launch */
     for (int e = threadIdx.x; e < k * P; e += blockDim.x) {
         const int c = e / P;
         if (!L.acc[c]) continue;
         const size_t r = (size_t)(i0 + c) * P + (e - c * P);
         L.th[r] = L.prop[e];
-        d.theta[(size_t)x * n * P + r] = L.prop[e];
+        __hip_atomic_store(d.theta + (size_t)x * n * P + r, L.prop[e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
 }
@@ -1226,7 +1228,11 @@ __host__ __device__ inline size_t phi_chain_lds_bytes(int P, int nblk) {
  * values), for one chain. Chain i's partners come from the other chain
  * groups, whose states do not change in this launch. Publishes
  * d.phi_ready[f*n + i] = iter after the state is stored. */
-__device__ __forceinline__ void phi_chain(const Dev &d, int f, int i, uint32_t iter, char *lds_raw) {
+__device__ __forceinline__ void wait_chains(const Dev &d, int f, int c_lo, int c_n, uint32_t iter,
+                                            bool acquire);
+
+__device__ __forceinline__ void phi_chain(const Dev &d, int f, int i, uint32_t iter, char *lds_raw,
+                                          bool chained) {
     const int n = d.nchain, P = d.P, P2 = 2 * P, G = d.G;
     const JobDev job = d.phi_jobs[f];
     const int S = job.nsubject, nblk = (S + kStatBlock - 1) / kStatBlock;
@@ -1238,6 +1244,10 @@ __device__ __forceinline__ void phi_chain(const Dev &d, int f, int i, uint32_t i
     int m0, m1;
     double U;
     partners(st, i, group_lo(g, n, G), group_lo(g + 1, n, G), n, m0, m1, U);
+    /* every chain group in one launch: the earlier groups' chains first (the
+     * partners among them are read after their step, with device-scope
+     * loads; the later groups' chains wait for this one before they change) */
+    if (chained && g > 0) wait_chains(d, f, 0, group_lo(g, n, G), iter, false);
 
     PhiLds L{};   /* the term helpers read hc, hd, sb */
     double *p = reinterpret_cast<double *>(lds_raw);
@@ -1266,15 +1276,15 @@ __device__ __forceinline__ void phi_chain(const Dev &d, int f, int i, uint32_t i
     if (nblk == 1) block_load(tbase + min(t, P - 1), nq, S, v);
     {
         Piece<double, 1> pxi{xi, d.phi + (off + i) * P2, P2};
-        Piece<double, 1> px0{x0, d.phi + (off + m0) * P2, P2};
-        Piece<double, 1> px1{x1, d.phi + (off + m1) * P2, P2};
         Piece<double, 1> ppc{pc, d.pcache + (off + i) * P2, P2};
         const int qq = min(t, P2 - 1), pq = min(t, P - 1);
         const int k = job.hprior_off + qq, kp = job.prior_off + pq;
         pxi.load();
-        px0.load();
-        px1.load();
         ppc.load();
+        const int qp = min(t, P2 - 1);
+        const double *r0 = d.phi + (off + m0) * P2, *r1 = d.phi + (off + m1) * P2;
+        const double v0 = chained ? __hip_atomic_load(r0 + qp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : r0[qp];
+        const double v1 = chained ? __hip_atomic_load(r1 + qp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : r1[qp];
         const double h0 = d.pr_p0[k], h1 = d.pr_p1[k], h2 = d.pr_lo[k], h3 = d.pr_up[k];
         const double h4 = d.scache[2 * k], h5 = d.scache[2 * k + 1];
         const int hd = d.pr_dist[k];
@@ -1282,9 +1292,15 @@ __device__ __forceinline__ void phi_chain(const Dev &d, int f, int i, uint32_t i
         const double lp0 = d.phi_lp[off + i];
         const int valid = d.pcache_valid[off + i];
         pxi.store();
-        px0.store();
-        px1.store();
         ppc.store();
+        if (t < P2) {
+            x0[qp] = v0;
+            x1[qp] = v1;
+        }
+        for (int r = t + bs; r < P2; r += bs) {
+            x0[r] = __hip_atomic_load(r0 + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            x1[r] = __hip_atomic_load(r1 + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
         if (t < P2) {
             double *h = L.hc + (size_t)qq * 6;
             h[0] = h0;
@@ -1436,6 +1452,31 @@ __device__ __forceinline__ void wait_chains(const Dev &d, int f, int c_lo, int c
     if (acquire) __threadfence();
 }
 
+/* Every chain group in one launch: after an item of (subject, group, range
+ * r) has stored its chains (device-scope stores, complete before the
+ * barrier), ready[r] = iteration. */
+__device__ __forceinline__ void publish_item(uint32_t *ready, int r, uint32_t iter) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(ready + r, iter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/* Waits (bounded, as wait_chains) until ready[0..nr) all equal iter. */
+__device__ __forceinline__ void wait_items(const Dev &d, const uint32_t *ready, int nr, uint32_t iter) {
+    for (int r = threadIdx.x; r < nr; r += blockDim.x) {
+        uint32_t spins = 0;
+        while (__hip_atomic_load(ready + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != iter) {
+            if (++spins == kSpinLimit ||
+                __hip_atomic_load(d.fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                atomicOr(d.fault, 1u);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(4);
+        }
+    }
+    __syncthreads();
+}
+
 /* Prior terms of proposals L.prop[0..k) for chains i0 + c under the
  * population state just published (read with device-scope loads): the
  * operations of proposal_stage1/2 (location, scale and cached logarithms per
@@ -1466,7 +1507,7 @@ __device__ __forceinline__ void late_prior(const Dev &d, const JobDev &job, cons
 
 /* ------------------------------------------------------------ sweep kernel */
 template <int MODE>
-__global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs a) {
+__global__ __launch_bounds__(kBlock, 2) void subject_sweep_kernel(Dev d, SweepArgs a) {
     extern __shared__ __attribute__((aligned(16))) char lds_raw[];
     if ((int)blockIdx.x < a.phi_wgs) {
         /* the population step of the launch's chains, concurrent with the
@@ -1474,12 +1515,13 @@ __global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs 
          * the priors of the hierarchical subjects' proposals): one workgroup
          * per (state, chain); whole-run steps by the first chain's workgroup
          * of the first group */
-        const int c_lo = group_lo(a.g_begin, d.nchain, d.G);
-        const int nc = group_lo(a.g_end, d.nchain, d.G) - c_lo;
+        const int gb = a.chained ? 0 : a.g_begin, ge = a.chained ? d.G : a.g_end;
+        const int c_lo = group_lo(gb, d.nchain, d.G);
+        const int nc = group_lo(ge, d.nchain, d.G) - c_lo;
         const int f = (int)blockIdx.x / nc, i = c_lo + (int)blockIdx.x % nc;
         const uint32_t it = *d.iter_base + a.iter_off;
-        if (!phi_full(d, d.phi_jobs[f], it)) phi_chain(d, f, i, it, lds_raw);
-        else if (a.g_begin == 0 && i == 0) phi_block(d, it, 1, a.phi_stats_lds, f, lds_raw, true);
+        if (!phi_full(d, d.phi_jobs[f], it)) phi_chain(d, f, i, it, lds_raw, a.chained != 0);
+        else if (gb == 0 && i == 0) phi_block(d, it, 1, a.phi_stats_lds, f, lds_raw, true);
         return;
     }
     const int ib = (int)blockIdx.x - a.phi_wgs;
@@ -1488,6 +1530,10 @@ __global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs 
     const int c0 = item.y >> 16, c1 = item.y & 0xFFFF;
     const int t = item.z >> 16, ntile = item.z & 0xFFFF;
     const int n = d.nchain, P = d.P, G = d.G;
+    /* the item's chain groups [g_begin, g_end) and chain range r */
+    const int g_begin = a.chained ? item.w >> 16 : a.g_begin;
+    const int g_end = a.chained ? g_begin + 1 : a.g_end;
+    const int rng = item.w & 0xFFFF;
     const SweepLds L = carve(lds_raw, n, P, a.kmax, d.n_cond * d.n_acc, a.chain_batch, a.cmax, a.tstage);
     /* the item's trials; staged in LDS when they fit */
     const int64_t s0 = d.subj_off[x], s1 = d.subj_off[x + 1];
@@ -1561,21 +1607,36 @@ __global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs 
     if (!migrate) {
         /* stage 1 of the first group overlaps the state loads */
         int lo, hi, i0;
-        const int k = group_k(a.g_begin, lo, hi, i0);
+        const int k = group_k(g_begin, lo, hi, i0);
         if (k > 0) proposal_stage1(d, job, L, st, lo, hi, i0, k, para, late);
     }
     __syncthreads();
     /* deferred store: the first launch of iteration i stores iteration i - 1 */
-    if (a.store_prev && c0 == 0 && t == 0 && iter > 1) store_subject(d, L.th, L.lp, L.ll, x, iter - 1);
+    if (a.store_prev && g_begin == 0 && c0 == 0 && t == 0 && iter > 1)
+        store_subject(d, L.th, L.lp, L.ll, x, iter - 1);
+    uint32_t *ready = a.chained ? d.item_ready + ((size_t)x * G) * a.rmax : nullptr;
+    if (a.chained && migrate && g_begin == 0 && c0 == 0 && t == 0) publish_item(ready, rng, iter);
+    if (a.chained && !migrate && g_begin > 0) {
+        /* the previous group's items of this subject first; then the rows
+         * of the earlier groups' chains, which they changed */
+        const int ng = group_lo(g_begin, n, G) - group_lo(g_begin - 1, n, G);
+        wait_items(d, ready + (size_t)(g_begin - 1) * a.rmax, (ng + a.kmax - 1) / a.kmax, iter);
+        const double *src = d.theta + (size_t)x * n * P;
+        for (int e = threadIdx.x; e < group_lo(g_begin, n, G) * P; e += blockDim.x)
+            L.th[e] = __hip_atomic_load(src + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+    }
 
     if (migrate) {
         const bool act = blocked && single ? a.sweep == 0 : true;
         /* with the population step in these launches, a hierarchical
          * subject migrates in the last group's launch, after every chain's
-         * population step */
-        const bool here = late ? a.g_end == G : a.g_begin == 0;
+         * population step (and, all groups in one launch, after the first
+         * item has stored the previous iteration) */
+        const bool here = late ? g_end == G : g_begin == 0;
         if (c0 == 0 && t == 0 && act && here) {
             const int para = (blocked && !single) ? a.sweep : -1;
+            if (late && a.chained) wait_items(d, ready, 1, iter);
             if (late) wait_chains(d, job.phi_index, 0, n, iter, true);
             subject_migration<MODE>(d, L, job, x, sd, para, a.cmax,
                                     ntile == 1 && a.tstage > 0 ? tv : TrialView{d.rt, d.tcode, 0});
@@ -1583,11 +1644,11 @@ __global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs 
     } else {
         const int j_lo = (int)((t_lo - s0) / kChunk);
         const int nj = (int)((t_hi - t_lo + kChunk - 1) / kChunk);
-        for (int g = a.g_begin; g < a.g_end; ++g) {
+        for (int g = g_begin; g < g_end; ++g) {
             int lo, hi, i0;
             const int k = group_k(g, lo, hi, i0);
             if (k <= 0) continue;
-            if (g != a.g_begin) {
+            if (g != g_begin) {
                 proposal_stage1(d, job, L, st, lo, hi, i0, k, para, late);
                 __syncthreads();
             }
@@ -1628,6 +1689,7 @@ __global__ __launch_bounds__(kBlock) void subject_sweep_kernel(Dev d, SweepArgs 
                 late_prior(d, job, L, i0, k);
             }
             accept_proposals(d, L, x, i0, k);
+            if (a.chained) publish_item(ready + (size_t)g * a.rmax, rng, iter);
         }
     }
 }

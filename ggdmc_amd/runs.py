@@ -148,12 +148,12 @@ class _Out:
 def make_config(nmc, nchain, thin=1, pop_migration_prob=0.0, sub_migration_prob=0.0,
                 gamma_precursor=2.38, rp=0.001, is_hblocked=False, is_pblocked=False,
                 precision=_lib.MIXED_BATCHED, sweep_groups=0, trials_per_tile=0,
-                chains_per_item=0, iter_offset=0):
+                chains_per_item=0, launch_mode=0, iter_offset=0):
     return _lib.Config(int(nmc), int(thin), int(nchain), float(pop_migration_prob),
                        float(sub_migration_prob), float(gamma_precursor), float(rp),
                        int(bool(is_hblocked)), int(bool(is_pblocked)), int(precision),
                        int(sweep_groups), int(trials_per_tile), int(chains_per_item),
-                       int(iter_offset))
+                       int(launch_mode), int(iter_offset))
 
 
 @dataclass
@@ -253,12 +253,15 @@ class Session:
 
     def decomposition(self):
         """(chains per item or -1 for one item per subject, trials per item or 0
-        for whole subjects, items of the largest launch)."""
+        for whole subjects, items of the largest launch, launch mode: 1 = one
+        subject launch per chain group, 2 = one for every group)."""
         k = C.c_int32()
         t = C.c_int32()
         m = C.c_int32()
-        check(_lib.lib().ggdmc_session_decomposition(self.h, C.byref(k), C.byref(t), C.byref(m)))
-        return k.value, t.value, m.value
+        lm = C.c_int32()
+        check(_lib.lib().ggdmc_session_decomposition(self.h, C.byref(k), C.byref(t), C.byref(m),
+                                                     C.byref(lm)))
+        return k.value, t.value, m.value, lm.value
 
     def download(self):
         """Returns per job {'phi': Posterior|None, 'subjects': [Posterior]}."""

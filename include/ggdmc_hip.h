@@ -65,6 +65,10 @@ typedef struct {
     int32_t chains_per_item; /* chains of a chain group per work item: 0 = all (autotuned
                                 when trials_per_tile is 0 too); -1 = one item per subject
                                 covering every group in one launch */
+    int32_t launch_mode;    /* subject launches per sweep: 0 = autotuned with the two above
+                               (else 1), 1 = one per chain group, 2 = one for every chain
+                               group (an item waits for the previous group's items of its
+                               subject); never changes results */
     uint64_t iter_offset;   /* Philox iteration offset (restart continues a stream) */
 } ggdmc_config;
 
@@ -167,11 +171,13 @@ void ggdmc_session_destroy(ggdmc_session *s);
 
 /* Work decomposition of a session's subject launches (after create, so after
  * autotuning): chains per item (-1: one item per subject over every group),
- * trials per item (0: whole subject) and the items of the largest launch.
- * The decomposition never changes results: log-likelihoods are summed in a
- * fixed chunk order whatever the split. */
+ * trials per item (0: whole subject), the items of the largest launch and the
+ * launch mode (1 or 2, as ggdmc_config.launch_mode). The decomposition never
+ * changes results: log-likelihoods are summed in a fixed chunk order whatever
+ * the split. */
 int ggdmc_session_decomposition(ggdmc_session *s, int32_t *chains_per_item,
-                                int32_t *trials_per_tile, int32_t *items_per_launch);
+                                int32_t *trials_per_tile, int32_t *items_per_launch,
+                                int32_t *launch_mode);
 
 const char *ggdmc_last_error(void);
 /* Number of visible gfx950 devices (0 without a GPU; never fails). */

@@ -175,21 +175,24 @@ def test_decomposition_never_changes_results(gpu, precision):
     datas = [fix.pop_data(s) for s in range(S)]
     prec = getattr(gpu, precision)
     ref = None
-    for k, tpt in ((-1, 0), (0, 0), (1, 0), (4, 128), (2, 256), (0, 384)):
+    # (chains per item, trials per item, launch mode: 1 = a launch per chain
+    # group, 2 = every group in one launch, later groups waiting for earlier)
+    for k, tpt, lm in ((-1, 0, 1), (0, 0, 1), (1, 0, 1), (4, 128, 1), (2, 256, 1), (0, 384, 1),
+                       (0, 0, 2), (1, 0, 2), (4, 128, 2), (2, 256, 2)):
         cfg = gpu.make_config(nmc=6, nchain=n, pop_migration_prob=0.2, sub_migration_prob=0.3,
                               precision=prec, sweep_groups=2, chains_per_item=k,
-                              trials_per_tile=tpt)
+                              trials_per_tile=tpt, launch_mode=lm)
         r = gpu.run(cfg, fix.model, fix.pop_prior, fix.hyper_prior, datas, (phi, plp, pll), subs,
                     seed=31)
         if ref is None:
             ref = r
             continue
-        assert np.array_equal(r["phi"].theta, ref["phi"].theta), (k, tpt)
-        assert np.array_equal(r["phi"].log_likelihoods, ref["phi"].log_likelihoods), (k, tpt)
+        assert np.array_equal(r["phi"].theta, ref["phi"].theta), (k, tpt, lm)
+        assert np.array_equal(r["phi"].log_likelihoods, ref["phi"].log_likelihoods), (k, tpt, lm)
         for a, b in zip(r["subject_theta"], ref["subject_theta"]):
-            assert np.array_equal(a.theta, b.theta), (k, tpt)
-            assert np.array_equal(a.log_likelihoods, b.log_likelihoods), (k, tpt)
-            assert np.array_equal(a.summed_log_prior, b.summed_log_prior), (k, tpt)
+            assert np.array_equal(a.theta, b.theta), (k, tpt, lm)
+            assert np.array_equal(a.log_likelihoods, b.log_likelihoods), (k, tpt, lm)
+            assert np.array_equal(a.summed_log_prior, b.summed_log_prior), (k, tpt, lm)
 
 
 def test_loglike_api_equals_sampler_values(gpu):
