@@ -85,7 +85,7 @@ struct Dev {
     const double *scache;       /* [prior entries][2]: the same for fixed priors (unif: log width) */
 
     uint32_t *stats_ctr;        /* [F] statistics workgroups finished */
-    uint32_t *tile_ctr;         /* [items] tiles of a (subject, chain range) finished */
+    uint32_t *tile_ctr;         /* [S][G][rmax] tiles of a (subject, group, chain range) finished */
     uint32_t *phi_ready;        /* [F][n] last iteration whose population step of a chain is stored */
     uint32_t *fault;            /* [1] nonzero: a wait for phi_ready gave up (results invalid) */
     uint32_t *item_ready;       /* [S][G][rmax] last iteration whose (subject, group, chain range)

@@ -562,11 +562,34 @@ void ggdmc_session::set_decomp(bool fused_new, int k_new, int tpt_new, bool chai
     kchain = fused ? ngmax : std::min(k_new > 0 ? k_new : ngmax, ngmax);
     lists.clear();
     list_n.clear();
-    size_t max_items = 1;
     int64_t max_chunks_item = 1;
     const int nl = fused ? 1 : G;
     std::vector<int4> all_items;   /* every group's items, in group order (chained) */
     rmax = 1;
+    /* XCD-aware order: workgroup w runs on XCD w % 8, so the items of subject
+     * x go to workgroups w = x (mod 8) where possible, and share that XCD's
+     * L2 for the subject's trials and state. The population workgroups come
+     * first in a launch when the population step runs inside it. */
+    constexpr int kXcd = 8;
+    const bool phi_in = d.n_phi > 0 && d.S > 0 && !fused && nsweep_sub == 1;
+    int64_t wg_base = phi_in && chained ? (int64_t)d.n_phi * n : 0;
+    auto xcd_order = [&](std::vector<int4> &v, int64_t base) {
+        std::vector<std::vector<int4>> b(kXcd);
+        for (const int4 &it : v) b[it.x % kXcd].push_back(it);
+        std::vector<size_t> pos(kXcd, 0);
+        std::vector<int4> out;
+        out.reserve(v.size());
+        for (size_t p = 0; p < v.size(); ++p) {
+            const int want = (int)((base + (int64_t)p) % kXcd);
+            int pick = -1;
+            for (int dx = 0; dx < kXcd && pick < 0; ++dx) {
+                const int c = (want + dx) % kXcd;
+                if (pos[c] < b[c].size()) pick = c;
+            }
+            out.push_back(b[pick][pos[pick]++]);
+        }
+        v.swap(out);
+    };
     for (int g = 0; g < nl; ++g) {
         const int ng = fused ? ngmax : (((g + 1) * n) / G - (g * n) / G);
         const int nch = fused ? 1 : (ng + kchain - 1) / kchain;
@@ -587,25 +610,29 @@ void ggdmc_session::set_decomp(bool fused_new, int k_new, int tpt_new, bool chai
         }
         rmax = std::max(rmax, nch);
         if (chained) {
+            xcd_order(items, wg_base);
+            wg_base += (int64_t)items.size();
             all_items.insert(all_items.end(), items.begin(), items.end());
             continue;
         }
+        xcd_order(items, phi_in ? (int64_t)d.n_phi * ng : 0);
         require(items.size() < ((size_t)1 << 31), "too many work items");
         list_n.push_back((int)items.size());
         lists.push_back(pool.upload(items, stream));
-        max_items = std::max(max_items, items.size());
     }
     if (chained) {
         require(all_items.size() < ((size_t)1 << 31), "too many work items");
         list_n.push_back((int)all_items.size());
         lists.push_back(pool.upload(all_items, stream));
-        max_items = std::max(max_items, all_items.size());
         const size_t nf = (size_t)std::max(d.S, 1) * G * rmax;
         d.item_ready = pool.alloc<uint32_t>(nf);
         HIPCHECK(hipMemsetAsync(d.item_ready, 0, sizeof(uint32_t) * nf, stream));
     }
-    d.tile_ctr = pool.alloc<uint32_t>(max_items);
-    HIPCHECK(hipMemsetAsync(d.tile_ctr, 0, sizeof(uint32_t) * max_items, stream));
+    {
+        const size_t nctr = (size_t)std::max(d.S, 1) * G * rmax;
+        d.tile_ctr = pool.alloc<uint32_t>(nctr);
+        HIPCHECK(hipMemsetAsync(d.tile_ctr, 0, sizeof(uint32_t) * nctr, stream));
+    }
     /* LDS: accumulator constants of chain_batch chains and cmax chunk sums
      * per chain at once, within 48 KB when possible */
     kmax = kchain;

@@ -1667,7 +1667,7 @@ __global__ __launch_bounds__(kBlock, 2) void subject_sweep_kernel(Dev d, SweepAr
                 /* the last-arriving tile of (subject, chain range) accepts */
                 __threadfence();
                 __syncthreads();
-                uint32_t *ctr = d.tile_ctr + (ib - t);
+                uint32_t *ctr = d.tile_ctr + ((size_t)x * G + g) * a.rmax + rng;
                 if (threadIdx.x == 0) {
                     const uint32_t old = atomicAdd(ctr, 1u);
                     L.flag[3] = old == (uint32_t)(ntile - 1);
