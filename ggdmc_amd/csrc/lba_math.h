@@ -107,9 +107,11 @@ GGDMC_FN double exp2_neg(double y) {
     const double k = std::rint(yc * 64.0);
     const double x = (yc - k * 0.015625) * kLn2;
     const int ki = (int)k;
-    const double x2 = x * x;
-    const double p = (1.0 + x) + x2 * ((0.5 + 0.16666666666666666 * x) +
-                                       x2 * (0.041666666666666664 + 0.008333333333333333 * x));
+    /* Horner: one constant per multiply-add, which GFX9's single constant-bus
+     * read per instruction takes straight from a scalar register */
+    const double p =
+        1.0 + x * (1.0 + x * (0.5 + x * (0.16666666666666666 + x * (0.041666666666666664 +
+                                                                  x * 0.008333333333333333))));
     const double r = std::ldexp(Ops::exp2_frac(ki & 63) * p, ki >> 6);
     return in ? r : (y == y ? 0.0 : y);
 }
@@ -125,13 +127,13 @@ GGDMC_FN void mixed_tail(double z, double &e, double &qa) {
     e = exp2_neg<Ops>(-0.5 * z * z * kLog2e);
     const double a = std::fabs(z);
     const double t = rcp64<Ops>(1.0 + kMillsK * a);
-    /* Estrin's scheme for sum_k c_k t^k, k = 0..11 */
-    const double t2 = t * t, t4 = t2 * t2, t8 = t4 * t4;
-    const double p0 = kMills[0] + kMills[1] * t, p1 = kMills[2] + kMills[3] * t;
-    const double p2 = kMills[4] + kMills[5] * t, p3 = kMills[6] + kMills[7] * t;
-    const double p4 = kMills[8] + kMills[9] * t, p5 = kMills[10] + kMills[11] * t;
-    const double q0 = p0 + p1 * t2, q1 = p2 + p3 * t2, q2 = p4 + p5 * t2;
-    const double h = (q0 + q1 * t4) + q2 * t8;
+    /* sum_k c_k t^k, k = 0..11, by Horner's rule: each multiply-add reads one
+     * coefficient (a scalar register), so no coefficient is first moved into
+     * vector registers; the two trials and two tails a lane evaluates give the
+     * dependency chains to interleave */
+    double h = kMills[11];
+#pragma unroll
+    for (int j = 10; j >= 0; --j) h = kMills[j] + t * h;
     qa = e * t * h;
 }
 
