@@ -98,11 +98,13 @@ def test_hierarchical_posterior_matches_reference_order(gpu):
     assert np.all(np.abs(rl - 1) < 0.25) and np.all(np.abs(rs_ - 1) < 0.25)
 
 
-def test_batched_job_equals_job_alone(gpu):
+@pytest.mark.parametrize("launch_mode", [1, 2])
+def test_batched_job_equals_job_alone(gpu, launch_mode):
     # a run's trajectory does not depend on the other runs of its batch
     w = workloads.make("c4", scale=1 / 16, ll_fn=_oracle_ll)    # 16 ragged subjects + 4 fits
     cfg = gpu.make_config(nmc=6, nchain=w.nchain, precision=gpu.MIXED_BATCHED, sweep_groups=2,
-                          trials_per_tile=256, sub_migration_prob=0.2, pop_migration_prob=0.2)
+                          trials_per_tile=256, sub_migration_prob=0.2, pop_migration_prob=0.2,
+                          launch_mode=launch_mode)
     together = gpu.run_batch(cfg, w.model, w.jobs)
     for k in (0, len(w.jobs) - 1):
         alone = gpu.run_batch(cfg, w.model, [w.jobs[k]])[0]
@@ -111,6 +113,30 @@ def test_batched_job_equals_job_alone(gpu):
             assert np.array_equal(a.log_likelihoods, b.log_likelihoods)
         if alone["phi"] is not None:
             assert np.array_equal(alone["phi"].theta, together[k]["phi"].theta)
+
+
+def test_mixed_batch_schedules_agree(gpu):
+    # hierarchical run + independent fits in one batch: a launch per chain group
+    # and one launch for every group, with whole or tiled subjects, give the
+    # same bits (the waits between groups only order the work)
+    w = workloads.make("c4", scale=1 / 16, ll_fn=_oracle_ll)
+    ref = None
+    for k, tpt, lm in ((0, 0, 1), (2, 0, 2), (0, 256, 2), (4, 128, 2)):
+        cfg = gpu.make_config(nmc=6, nchain=w.nchain, precision=gpu.MIXED_BATCHED, sweep_groups=2,
+                              chains_per_item=k, trials_per_tile=tpt, launch_mode=lm,
+                              sub_migration_prob=0.2, pop_migration_prob=0.2)
+        res = gpu.run_batch(cfg, w.model, w.jobs)
+        if ref is None:
+            ref = res
+            continue
+        for a, b in zip(res, ref):
+            for x, y in zip(a["subjects"], b["subjects"]):
+                assert np.array_equal(x.theta, y.theta), (k, tpt, lm)
+                assert np.array_equal(x.log_likelihoods, y.log_likelihoods), (k, tpt, lm)
+                assert np.array_equal(x.summed_log_prior, y.summed_log_prior), (k, tpt, lm)
+            if a["phi"] is not None:
+                assert np.array_equal(a["phi"].theta, b["phi"].theta), (k, tpt, lm)
+                assert np.array_equal(a["phi"].log_likelihoods, b["phi"].log_likelihoods), (k, tpt, lm)
 
 
 def test_scale_workload_full_size(gpu):
