@@ -90,6 +90,8 @@ struct Dev {
     uint32_t *fault;            /* [1] nonzero: a wait for phi_ready gave up (results invalid) */
     uint32_t *item_ready;       /* [S][G][rmax] last iteration whose (subject, group, chain range)
                                    step is stored (all groups in one launch) */
+    uint32_t *item_staged;      /* [S][G][rmax] last iteration whose (subject, group, chain range)
+                                   item has read the subject's rows */
     unsigned long long *evals;  /* [S] trial-likelihood evaluations per subject */
     const uint32_t *iter_base;  /* device iteration base (graph replays) */
 
@@ -122,6 +124,7 @@ struct SweepArgs {
     int32_t chained;      /* every chain group in this launch: items carry their group, an
                              item waits for the previous group's items of its subject */
     int32_t rmax;         /* chain ranges per group (item_ready stride) */
+    int32_t tiles;        /* some subject's trials are split over several items */
 };
 
 } // namespace ggdmc

@@ -133,6 +133,7 @@ struct ggdmc_session {
      * of its own kernel: */
     bool overlap = false;
     bool chained = false;         /* every chain group in one subject launch */
+    bool tiled = false;           /* some subject's trials split over several items */
     int rmax = 1;                 /* chain ranges per group (item_ready stride) */
     int ovl_stats_lds = 0;
     size_t ovl_lds = 0;           /* LDS of those launches: max(sweep, population) */
@@ -557,6 +558,7 @@ void ggdmc_session::set_decomp(bool fused_new, int k_new, int tpt_new, bool chai
     const int ngmax = (n + G - 1) / G;
     fused = fused_new;
     chained = chained_new;
+    tiled = false;
     tpt = tpt_new;
     d.tpt = tpt;
     kchain = fused ? ngmax : std::min(k_new > 0 ? k_new : ngmax, ngmax);
@@ -598,6 +600,7 @@ void ggdmc_session::set_decomp(bool fused_new, int k_new, int tpt_new, bool chai
             const int64_t ntr = subj_ntr[x];
             const int64_t nt = (tpt == 0 || ntr <= tpt) ? 1 : (ntr + tpt - 1) / tpt;
             require(nt <= 65535, "trials_per_tile too small for a subject's trial count");
+            tiled = tiled || nt > 1;
             require((int64_t)nch * nt < (int64_t)1 << 30, "too many work items per subject");
             for (int ch = 0; ch < nch; ++ch) {
                 const int c0 = ch * kchain;
@@ -627,6 +630,8 @@ void ggdmc_session::set_decomp(bool fused_new, int k_new, int tpt_new, bool chai
         const size_t nf = (size_t)std::max(d.S, 1) * G * rmax;
         d.item_ready = pool.alloc<uint32_t>(nf);
         HIPCHECK(hipMemsetAsync(d.item_ready, 0, sizeof(uint32_t) * nf, stream));
+        d.item_staged = pool.alloc<uint32_t>(nf);
+        HIPCHECK(hipMemsetAsync(d.item_staged, 0, sizeof(uint32_t) * nf, stream));
     }
     {
         const size_t nctr = (size_t)std::max(d.S, 1) * G * rmax;
@@ -686,7 +691,7 @@ void ggdmc_session::enqueue_iteration(uint32_t off, std::vector<hipEvent_t> *ev)
             const int nc = chained ? d.nchain : ((g + 1) * d.nchain) / d.G - (g * d.nchain) / d.G;
             SweepArgs a{off, sw, (fused || chained) ? 0 : g, (fused || chained) ? d.G : g + 1, (int)first,
                         chain_batch, kmax, cmax, tstage, list_n[g], lists[g],
-                        with_phi ? d.n_phi * nc : 0, ovl_stats_lds, (int)chained, rmax};
+                        with_phi ? d.n_phi * nc : 0, ovl_stats_lds, (int)chained, rmax, (int)tiled};
             hipEvent_t e0 = nullptr, e1 = nullptr;
             if (ev) {
                 HIPCHECK(hipEventCreate(&e0));
@@ -826,9 +831,11 @@ void ggdmc_session::autotune() {
             /* the iterations run again: their population steps are not done */
             HIPCHECK(hipMemsetAsync(d.phi_ready, 0, sizeof(uint32_t) * nf, stream));
         }
-        if (chained)
-            HIPCHECK(hipMemsetAsync(d.item_ready, 0, sizeof(uint32_t) * (size_t)std::max(d.S, 1) * d.G * rmax,
-                                    stream));
+        if (chained) {
+            const size_t nf = (size_t)std::max(d.S, 1) * d.G * rmax;
+            HIPCHECK(hipMemsetAsync(d.item_ready, 0, sizeof(uint32_t) * nf, stream));
+            HIPCHECK(hipMemsetAsync(d.item_staged, 0, sizeof(uint32_t) * nf, stream));
+        }
         HIPCHECK(hipStreamSynchronize(stream));
     };
     save();

@@ -1615,12 +1615,40 @@ __global__ __launch_bounds__(kBlock, 2) void subject_sweep_kernel(Dev d, SweepAr
     if (a.store_prev && g_begin == 0 && c0 == 0 && t == 0 && iter > 1)
         store_subject(d, L.th, L.lp, L.ll, x, iter - 1);
     uint32_t *ready = a.chained ? d.item_ready + ((size_t)x * G) * a.rmax : nullptr;
+    uint32_t *staged = a.chained ? d.item_staged + ((size_t)x * G) * a.rmax : nullptr;
+    /* this item has read the subject's rows: a later group may change them */
+    if (a.chained && t == 0 && threadIdx.x == 0)
+        __hip_atomic_store(staged + (size_t)g_begin * a.rmax + rng, iter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (a.chained && migrate && g_begin == 0 && c0 == 0 && t == 0) publish_item(ready, rng, iter);
     if (a.chained && !migrate && g_begin > 0) {
         /* the previous group's items of this subject first; then the rows
          * of the earlier groups' chains, which they changed */
-        const int ng = group_lo(g_begin, n, G) - group_lo(g_begin - 1, n, G);
-        wait_items(d, ready + (size_t)(g_begin - 1) * a.rmax, (ng + a.kmax - 1) / a.kmax, iter);
+        const int glo = group_lo(g_begin - 1, n, G);
+        const int nr = (group_lo(g_begin, n, G) - glo + a.kmax - 1) / a.kmax;
+        if (G == 2 && !a.tiles) {
+            /* two groups, whole subjects: every group-0 item of the subject
+             * has read its rows, and the items holding this item's partners
+             * (all in group 0) are done */
+            wait_items(d, staged, nr, iter);
+            int lo1, hi1, i01;
+            const int k1 = group_k(g_begin, lo1, hi1, i01);
+            for (int e = threadIdx.x; e < 2 * k1; e += blockDim.x) {
+                const int m = (e & 1) ? L.m1[e >> 1] : L.m0[e >> 1];
+                const uint32_t *f = ready + (m - glo) / a.kmax;
+                uint32_t spins = 0;
+                while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != iter) {
+                    if (++spins == kSpinLimit ||
+                        __hip_atomic_load(d.fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                        atomicOr(d.fault, 1u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(4);
+                }
+            }
+            __syncthreads();
+        } else {
+            wait_items(d, ready + (size_t)(g_begin - 1) * a.rmax, nr, iter);
+        }
         const double *src = d.theta + (size_t)x * n * P;
         for (int e = threadIdx.x; e < group_lo(g_begin, n, G) * P; e += blockDim.x)
             L.th[e] = __hip_atomic_load(src + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
