@@ -139,8 +139,8 @@ __device__ __forceinline__ void chunk_sums(const Dev &d, const TrialView &tv, in
                 a = h1 ? ma : 0.0;
                 b = h2 ? mb : 0.0;
                 /* rare: ill-conditioned trials take the reference formula */
-                if (h1 && ill1) a = naive_trial_term<DevOps>(k1, d.n_acc, code1 & 7, rt1);
-                if (h2 && ill2) b = naive_trial_term<DevOps>(k2, d.n_acc, code2 & 7, rt2);
+                if (__builtin_expect(h1 && ill1, 0)) a = naive_trial_term<DevOps>(k1, d.n_acc, code1 & 7, rt1);
+                if (__builtin_expect(h2 && ill2, 0)) b = naive_trial_term<DevOps>(k2, d.n_acc, code2 & 7, rt2);
             } else {
                 if (h1) a = naive_trial_term<DevOps>(k1, d.n_acc, code1 & 7, rt1);
                 if (h2) b = naive_trial_term<DevOps>(k2, d.n_acc, code2 & 7, rt2);
@@ -1509,7 +1509,9 @@ __device__ __forceinline__ void late_prior(const Dev &d, const JobDev &job, cons
 template <int MODE>
 __global__ __launch_bounds__(kBlock, 2) void subject_sweep_kernel(Dev d, SweepArgs a) {
     extern __shared__ __attribute__((aligned(16))) char lds_raw[];
-    if ((int)blockIdx.x < a.phi_wgs) {
+    /* (the population workgroups are few: their code is laid out after the
+     * subjects' path) */
+    if (__builtin_expect((int)blockIdx.x < a.phi_wgs, 0)) {
         /* the population step of the launch's chains, concurrent with the
          * subjects' proposals and likelihoods (they depend on it only through
          * the priors of the hierarchical subjects' proposals): one workgroup
@@ -1655,7 +1657,7 @@ __global__ __launch_bounds__(kBlock, 2) void subject_sweep_kernel(Dev d, SweepAr
         __syncthreads();
     }
 
-    if (migrate) {
+    if (__builtin_expect(migrate, 0)) {
         const bool act = blocked && single ? a.sweep == 0 : true;
         /* with the population step in these launches, a hierarchical
          * subject migrates in the last group's launch, after every chain's
@@ -1691,7 +1693,7 @@ __global__ __launch_bounds__(kBlock, 2) void subject_sweep_kernel(Dev d, SweepAr
                 for (int c = 0; c < k; ++c) na += L.active[c];
                 atomicAdd(d.evals + x, (unsigned long long)na * (unsigned long long)(t_hi - t_lo));
             }
-            if (ntile > 1) {
+            if (__builtin_expect(ntile > 1, 0)) {
                 /* the last-arriving tile of (subject, chain range) accepts */
                 __threadfence();
                 __syncthreads();
