@@ -15,7 +15,9 @@ Measured on the same inputs, all resident in HBM before the timed region:
   roofline       of the subject sweep kernel (dominant): VALU issue bound,
                  algorithmic slots per evaluation x evaluations per launch /
                  launch time (HIP events on the launch stream)
-  cpu_baseline   the FP64 oracle restatement of the reference sampler, one core
+  cpu_baseline   the FP64 oracle restatement of the reference sampler: one run
+                 on one core (value), and as many independent runs as host
+                 cores (at most 16) at once (aggregate, run-iterations/s)
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3]
 """
@@ -61,10 +63,14 @@ def parse():
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--workload", default="c3")
+    ap.add_argument("--runs", type=int, default=1,
+                    help="independent runs of the workload in one batch (Philox keys seed + r)")
     ap.add_argument("--plain-steps", type=int, default=0,
                     help="iterations of the plain FP64 path (0: 20, or 3 for c4 / c5)")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-cores", type=int, default=0,
+                    help="threads for the aggregate CPU baseline (0: the host cores, at most 16)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--profile-iters", type=int, default=20)
     ap.add_argument("--decomp", default="",
@@ -89,10 +95,33 @@ def calibrate_copy():
     print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
 
 
-def cpu_baseline(w, budget_s):
-    """Oracle (FP64, single thread) on the same workload, every job of it one
-    after the other, for a bounded number of iterations; returns iterations/s
-    and the sample description."""
+def host_cores():
+    """Cores this process may run on, capped at 16 (the host share of one GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, budget_s, cores):
+    """Oracle (FP64, one thread per run) on the same workload, every job of it
+    one after the other, for a bounded number of iterations. First one run on
+    one core, then `cores` independent runs (seeds offset by the run index) on
+    as many threads at once, the way the reference runs one sampler per forked
+    process. Returns (per-core iter/s, n, seconds, aggregate iter/s, n_agg,
+    wall seconds)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     from ggdmc_amd import _lib
@@ -101,10 +130,10 @@ def cpu_baseline(w, budget_s):
     def opr(p):
         return O.Prior(p.dist, p.p0, p.p1, p.lower, p.upper)
 
-    def one(j, n_iter):
+    def one(j, n_iter, run):
         cfg = O.config(nmc=n_iter + 1, nchain=w.nchain, nparameter_de=j.nparameter_de,
                        sub_migration_prob=w.sub_migration_prob,
-                       pop_migration_prob=w.pop_migration_prob, seed=j.seed)
+                       pop_migration_prob=w.pop_migration_prob, seed=j.seed + run)
         t = time.perf_counter()
         if j.kind == _lib.KIND_HIER:
             datas = j.subjects
@@ -119,13 +148,20 @@ def cpu_baseline(w, budget_s):
             O.run_subject(cfg, om, opr(j.p_prior), j.subject.rt, j.subject.cell_off, *j.start)
         return time.perf_counter() - t
 
-    def all_jobs(n_iter):
-        return sum(one(j, n_iter) for j in w.jobs)
+    def all_jobs(n_iter, run=0):
+        return sum(one(j, n_iter, run) for j in w.jobs)
 
     t1 = all_jobs(1)
     n = max(2, int(budget_s / max(t1, 1e-6)))
     dt = all_jobs(n)
-    return n / dt, n, dt
+    # The oracle's C calls release the GIL, so threads run on separate cores.
+    from concurrent.futures import ThreadPoolExecutor
+    n_agg = max(2, int(n * 0.6))
+    t = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=cores) as ex:
+        list(ex.map(lambda r: all_jobs(n_agg, r), range(cores)))
+    wall = time.perf_counter() - t
+    return n / dt, n, dt, cores * n_agg / wall, n_agg, wall
 
 
 def main():
@@ -144,7 +180,8 @@ def main():
         sys.exit(1)
 
     t_build = time.time()
-    w = workloads.make(a.workload)
+    w_one = workloads.make(a.workload)
+    w = workloads.replicate(w_one, a.runs)
     n_acc = w.model.n_acc
     total = a.warmup + a.steps + a.profile_iters
     thin = max(1, total // 1000)
@@ -235,11 +272,19 @@ def main():
 
     cpu = None
     if not a.no_cpu:
-        cps, n_cpu, dt = cpu_baseline(w, a.cpu_seconds)
+        cores = a.cpu_cores or host_cores()
+        cps, n_cpu, dt, agg, n_agg, wall = cpu_baseline(w_one, a.cpu_seconds, cores)
         cpu = {"value": round(cps, 4), "unit": "iter/s", "cores": 1, "kind": "port",
                "sample": "%d iterations of the same %s workload (all %d runs) in %.1f s "
-                         "(FP64 oracle, one thread)" % (n_cpu, w.name, len(w.jobs), dt)}
+                         "(FP64 oracle, one thread)" % (n_cpu, w_one.name, len(w_one.jobs), dt),
+               "cpu_model": cpu_model(),
+               "aggregate": {"value": round(agg, 4), "unit": "run-iter/s", "cores": cores,
+                             "sample": "%d independent runs x %d iterations each, one per "
+                                       "thread, in %.1f s wall" % (cores, n_agg, wall)}}
         print("[bench] cpu oracle: %s" % cpu["sample"], file=sys.stderr, flush=True)
+        print("[bench] cpu oracle, %d cores: %s: %.3f run-iter/s (%.3f per core)"
+              % (cores, cpu["aggregate"]["sample"], agg, agg / cores),
+              file=sys.stderr, flush=True)
 
     out = {
         "metric": METRIC, "value": round(iters_per_s, 2), "unit": "iter/s", "n_gpus": 1,
@@ -249,11 +294,14 @@ def main():
         "config": {"workload": "%s: %s" % (w.name, w.description), "subjects": w.n_subjects,
                    "trials": w.n_trials, "chains": w.nchain, "sweep_groups": 2,
                    "chains_per_item": kci, "trials_per_item": tpt, "items_per_launch": nitems,
-                   "launch_mode": lmode},
+                   "launch_mode": lmode, "runs": a.runs},
+        "run_iter_per_s": round(iters_per_s * a.runs, 2),
         "trial_evals_per_s": round(evals_per_s, 1),
         "plain_fp64_iter_per_s": round(plain, 3) if plain else None,
         "speedup_vs_plain_fp64": round(iters_per_s / plain, 2) if plain else None,
         "speedup_vs_cpu": round(iters_per_s / cpu["value"], 1) if cpu else None,
+        "speedup_vs_cpu_all_cores": (round(iters_per_s * a.runs / cpu["aggregate"]["value"], 1)
+                                     if cpu else None),
         "max_rel_dLL": worst, "dLL_states_checked": n_cmp,
         "download": {"ms": round(dl_ms, 2), "MB": round(dl_bytes / 1e6, 2), "columns": nmc},
         "roofline": roofline, "cpu_baseline": cpu,

@@ -13,7 +13,7 @@ Designs, priors and generating values follow the reference's data generators
 single-subject priors, tnorm(lower 0) population distributions, uniform(0, 10)
 hyper priors.
 """
-from dataclasses import dataclass
+from dataclasses import dataclass, replace
 
 import numpy as np
 
@@ -202,3 +202,15 @@ def make(name, scale=1.0, ll_fn=None):
         return Workload("c5", "hierarchical 4-acc LBA, %d subjects x %d trials, 11 params, "
                         "33 chains" % (S, T), m, [job], 33, 5000)
     raise ValueError("unknown workload " + name)
+
+
+def replicate(w, runs):
+    """The workload `runs` times over in one batch: every job repeated with
+    the same data and start state under Philox keys seed + r (r = 0..runs-1),
+    the way independent runs of one model are launched side by side."""
+    if runs <= 1:
+        return w
+    jobs = [replace(j, seed=j.seed + r) for r in range(runs) for j in w.jobs]
+    return Workload("%sx%d" % (w.name, runs), "%d independent runs of %s" % (runs, w.description),
+                    w.model, jobs, w.nchain, w.iterations, w.sub_migration_prob,
+                    w.pop_migration_prob)

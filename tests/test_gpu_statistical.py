@@ -139,6 +139,26 @@ def test_mixed_batch_schedules_agree(gpu):
                 assert np.array_equal(a["phi"].log_likelihoods, b["phi"].log_likelihoods), (k, tpt, lm)
 
 
+@pytest.mark.parametrize("launch_mode", [1, 2])
+def test_batch_of_hierarchical_runs_equals_each_alone(gpu, launch_mode):
+    # three hierarchical runs (same data, different Philox keys) in one batch:
+    # each gives the bits it gives alone, phi and subjects, with migration
+    w = workloads.replicate(workloads.make("c3", scale=1 / 8, ll_fn=_oracle_ll), 3)
+    cfg = gpu.make_config(nmc=6, nchain=w.nchain, precision=gpu.MIXED_BATCHED, sweep_groups=2,
+                          sub_migration_prob=0.2, pop_migration_prob=0.2,
+                          launch_mode=launch_mode)
+    together = gpu.run_batch(cfg, w.model, w.jobs)
+    for k in range(len(w.jobs)):
+        alone = gpu.run_batch(cfg, w.model, [w.jobs[k]])[0]
+        assert np.array_equal(alone["phi"].theta, together[k]["phi"].theta)
+        assert np.array_equal(alone["phi"].log_likelihoods, together[k]["phi"].log_likelihoods)
+        for a, b in zip(alone["subjects"], together[k]["subjects"]):
+            assert np.array_equal(a.theta, b.theta)
+            assert np.array_equal(a.log_likelihoods, b.log_likelihoods)
+    # different keys give different trajectories
+    assert not np.array_equal(together[0]["phi"].theta, together[1]["phi"].theta)
+
+
 def test_scale_workload_full_size(gpu):
     # 1024 subjects x 2048 trials, 4 accumulators: every stored state's
     # log-likelihood within the per-proposal tolerance of FP64
