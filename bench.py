@@ -52,7 +52,7 @@ W_ALG = {2: 130.0, 4: 255.0}
 W_ISSUED = {"c3": 660.0, "c5": 824.0}
 # HBM bytes per launch of the subject sweep kernel: rocprofv3 FETCH_SIZE x 2.0
 # + WRITE_SIZE x 1.0 (factors of a 256 MiB copy kernel; profiles/).
-TRAFFIC = {"c3": 2.678e6}
+TRAFFIC = {"c3": 2.678e6, "c5": 4.244e7}
 # 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
 VALU_PEAK_TSLOTS = 256 * 4 * 16 * 2.4e9 / 1e12
 
