@@ -89,10 +89,15 @@ struct Pool {
         ptrs.push_back(p);
         return static_cast<T *>(p);
     }
+    /* The copy has completed when this returns: callers pass host vectors
+     * that may go out of scope right after. */
     template <class T>
     T *upload(const std::vector<T> &v, hipStream_t st) {
         T *p = alloc<T>(v.size());
-        if (!v.empty()) HIPCHECK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+        if (!v.empty()) {
+            HIPCHECK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+            HIPCHECK(hipStreamSynchronize(st));
+        }
         return p;
     }
     ~Pool() {
@@ -874,6 +879,11 @@ static int fail(const std::exception &e) {
     return GGDMC_EINVAL;
 }
 
+static int invalid(const char *msg) {
+    g_error = msg;
+    return GGDMC_EINVAL;
+}
+
 /* ================================================================= C ABI */
 extern "C" {
 
@@ -943,7 +953,7 @@ int ggdmc_session_advance(ggdmc_session *s, int32_t n_iter) {
 }
 
 int ggdmc_session_elapsed_ms(ggdmc_session *s, float *ms) {
-    if (!s || !ms) return GGDMC_EINVAL;
+    if (!s || !ms) return invalid("session / ms is null");
     *ms = s->last_ms;
     return GGDMC_OK;
 }
@@ -951,7 +961,7 @@ int ggdmc_session_elapsed_ms(ggdmc_session *s, float *ms) {
 int ggdmc_session_decomposition(ggdmc_session *s, int32_t *chains_per_item,
                                 int32_t *trials_per_tile, int32_t *items_per_launch,
                                 int32_t *launch_mode) {
-    if (!s) return GGDMC_EINVAL;
+    if (!s) return invalid("session is null");
     if (launch_mode) *launch_mode = s->chained ? 2 : 1;
     if (chains_per_item) *chains_per_item = s->fused ? -1 : s->kchain;
     if (trials_per_tile) *trials_per_tile = s->tpt;
@@ -961,7 +971,7 @@ int ggdmc_session_decomposition(ggdmc_session *s, int32_t *chains_per_item,
 }
 
 int ggdmc_session_trial_evals(ggdmc_session *s, double *evals) {
-    if (!s || !evals) return GGDMC_EINVAL;
+    if (!s || !evals) return invalid("session / evals is null");
     try {
         const size_t ns = (size_t)std::max(s->d.S, 1);
         std::vector<unsigned long long> v(ns);
@@ -978,7 +988,7 @@ int ggdmc_session_trial_evals(ggdmc_session *s, double *evals) {
 
 int ggdmc_session_profile(ggdmc_session *s, int32_t n_iter, double *subject_kernel_ms,
                           int32_t *n_launches) {
-    if (!s || !subject_kernel_ms || n_iter < 1) return GGDMC_EINVAL;
+    if (!s || !subject_kernel_ms || n_iter < 1) return invalid("bad session / output / n_iter");
     try {
         if (s->mode == GGDMC_FP64_PLAIN || s->d.S == 0) {
             g_error = "profile needs a batched session with subjects";
@@ -1013,7 +1023,7 @@ int ggdmc_session_profile(ggdmc_session *s, int32_t n_iter, double *subject_kern
 }
 
 int ggdmc_session_download(ggdmc_session *s) {
-    if (!s) return GGDMC_EINVAL;
+    if (!s) return invalid("session is null");
     try {
         const size_t nmc = (size_t)s->cfg.nmc, n = (size_t)s->d.nchain, P = (size_t)s->d.P;
         HIPCHECK(hipStreamSynchronize(s->stream));
