@@ -208,3 +208,72 @@ def test_loglike_api_equals_sampler_values(gpu):
     moved = post.log_likelihoods[:, -1] != start[2]
     assert moved.any()
     assert np.array_equal(ll[moved], post.log_likelihoods[moved, -1])
+
+
+def test_three_chains_follow_oracle(gpu):
+    # the fewest chains the reference accepts (de.cpp:4-28): each chain's two
+    # partners are the other two; batched with three groups and plain
+    fix = fx.load("lba_data6")
+    n = 3
+    start = fx.near_start(fix, n, seed=13)
+    for prec in (gpu.FP64_BATCHED, gpu.FP64_PLAIN):
+        cfg = _cfg(gpu, nchain=n, precision=prec, sweep_groups=3, nmc=8, sub_migration_prob=0.3)
+        post = gpu.run_subject(cfg, fix.model, fix.sub_prior, fix.sub_data, start, seed=9032)
+        ocfg = _ocfg(fix, cfg)
+        oth, olp, oll = O.run_subject(ocfg, fix.omodel, fix.oprior, fix.sub_data.rt,
+                                      fix.sub_data.cell_off, *start)
+        _same(post, oth, olp, oll)
+    with pytest.raises(gpu.GgdmcError):
+        gpu.run_subject(_cfg(gpu, nchain=n, precision=gpu.FP64_BATCHED, sweep_groups=2),
+                        fix.model, fix.sub_prior, fix.sub_data, start, seed=1)
+
+
+def test_long_subject_follows_oracle(gpu):
+    # 100,608 trials (each cell of the fixture repeated 131 times): 786 chunks
+    # of 128 trials, more than the 682 chunk sums per chain an item holds in
+    # LDS with 6 chains per item, so the sums run in batches
+    fix = fx.load("lba_data6")
+    d0 = fix.sub_data
+    cells = [np.tile(d0.rt[d0.cell_off[c]:d0.cell_off[c + 1]], 131)
+             for c in range(len(d0.cell_off) - 1)]
+    data = fx.SubjectData.from_cells(cells)
+    assert len(data.rt) == 100608
+    n = 12
+    start = fx.near_start(fix, n, seed=17, data=data)
+    ocfg = None
+    for k in (6, 0):
+        cfg = _cfg(gpu, nchain=n, precision=gpu.FP64_BATCHED, sweep_groups=2, nmc=3,
+                   chains_per_item=k, launch_mode=1)
+        post = gpu.run_subject(cfg, fix.model, fix.sub_prior, data, start, seed=9032)
+        if ocfg is None:
+            ocfg = _ocfg(fix, cfg)
+            ocfg.sweep_groups = 2
+            ref = O.run_subject(ocfg, fix.omodel, fix.oprior, data.rt, data.cell_off, *start)
+        _same(post, *ref)
+
+
+def test_hier_with_an_empty_subject_follows_oracle(gpu):
+    # a subject with no trials contributes log-likelihood 0: its chains sample
+    # the population distribution; the other subjects are unaffected
+    fix = fx.load("lba_data6")
+    S, n = 3, 10
+    (phi, plp, pll), subs = fx.hier_start(fix, S, n, seed=19)
+    empty = fx.SubjectData(np.zeros(0), np.zeros(len(fix.sub_data.cell_off), np.int64))
+    datas = [fix.pop_data(0), empty, fix.pop_data(2)]
+    subs[1] = (subs[1][0], subs[1][1], np.zeros(n))
+    for prec, G in ((gpu.FP64_BATCHED, 2), (gpu.FP64_PLAIN, n)):
+        cfg = gpu.make_config(nmc=6, nchain=n, pop_migration_prob=0.3, sub_migration_prob=0.2,
+                              precision=prec, sweep_groups=G)
+        r = gpu.run(cfg, fix.model, fix.pop_prior, fix.hyper_prior, datas, (phi, plp, pll), subs,
+                    seed=555)
+        ocfg = _ocfg(fix, cfg, nparameter_de=2 * fix.P, seed=555)
+        ocfg.sweep_groups = G
+        rt = np.concatenate([d.rt for d in datas])
+        so = np.concatenate([[0], np.cumsum([len(d.rt) for d in datas])])
+        co = np.stack([d.cell_off for d in datas])
+        o = O.run_hier(ocfg, fix.omodel, fx.oprior(fix.pop_prior), fx.oprior(fix.hyper_prior), rt,
+                       so, co, phi, plp, pll, np.stack([s[0] for s in subs]),
+                       np.stack([s[1] for s in subs]), np.stack([s[2] for s in subs]))
+        _same_hier(r, o)
+        assert np.all(r["subject_theta"][1].log_likelihoods == 0.0)
+        assert not np.array_equal(r["subject_theta"][1].theta[:, :, -1], subs[1][0])
