@@ -111,7 +111,7 @@ struct SweepArgs {
     uint32_t iter_off;    /* iteration = *iter_base + iter_off */
     int32_t sweep;        /* blocked parameter index or 0 */
     int32_t g_begin, g_end;
-    int32_t store_prev;   /* first launch of the iteration: store iteration - 1 if due */
+    int32_t store_prev;   /* first sweep's launches: store iteration - 1 if due (each item its own chains) */
     int32_t chain_batch;  /* chains whose accumulator constants fit in LDS at once */
     int32_t kmax;         /* most chains one item proposes */
     int32_t cmax;         /* chunk sums per chain held in LDS at once */

@@ -690,11 +690,12 @@ void ggdmc_session::enqueue_iteration(uint32_t off, std::vector<hipEvent_t> *ev)
     const int ngl = (fused || chained) ? 1 : d.G;
     for (int sw = 0; sw < nsweep_sub; ++sw) {
         for (int g = 0; g < ngl; ++g) {
-            const bool first = sw == 0 && g == 0;
+            /* every group's launch of the first sweep stores its own chains */
+            const bool store = sw == 0;
             const bool with_phi = overlap && sw == 0;
             /* chains of the launch: group g, or every group */
             const int nc = chained ? d.nchain : ((g + 1) * d.nchain) / d.G - (g * d.nchain) / d.G;
-            SweepArgs a{off, sw, (fused || chained) ? 0 : g, (fused || chained) ? d.G : g + 1, (int)first,
+            SweepArgs a{off, sw, (fused || chained) ? 0 : g, (fused || chained) ? d.G : g + 1, (int)store,
                         chain_batch, kmax, cmax, tstage, list_n[g], lists[g],
                         with_phi ? d.n_phi * nc : 0, ovl_stats_lds, (int)chained, rmax, (int)tiled};
             hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -712,7 +713,7 @@ void ggdmc_session::enqueue_iteration(uint32_t off, std::vector<hipEvent_t> *ev)
 }
 
 /* Stores the subject states of the last iteration run (deferred in the
- * kernels to the next iteration's first launch) when that column is due. */
+ * kernels to the next iteration's first-sweep launches) when that column is due. */
 void ggdmc_session::flush_store() {
     const uint32_t it = next_iter - 1;
     if (it >= 1 && it % (uint32_t)cfg.thin == 0 && d.S > 0)

@@ -7,8 +7,8 @@
  *                        canonical chunk sums, Metropolis accept (:81-108) by the
  *                        item itself or by the last-arriving tile; migration ring
  *                        (src/de.cpp:615-665 / :157-199) and the deferred
- *                        thinned store of the previous iteration by the
- *                        subject's first item.
+ *                        thinned store of the previous iteration, every item
+ *                        storing the chains it updates.
  *  phi_kernel            per-(chain, parameter) sufficient statistics of the
  *                        subject states, then the population-level step
  *                        (src/de.cpp:385-465, :467-565) with sumloghlike
@@ -610,18 +610,19 @@ __device__ __forceinline__ void subject_migration(const Dev &d, const SweepLds &
     }
 }
 
-/* Thinned store of subject x's state (src: LDS copy or global state). */
-__device__ __forceinline__ void store_subject(const Dev &d, const double *th, const double *lp, const double *ll,
-                              int x, uint32_t iter) {
+/* Thinned store of chains [i0, i0 + k) of subject x (th, lp, ll: the whole
+ * subject's rows, in LDS). */
+__device__ __forceinline__ void store_chains(const Dev &d, const double *th, const double *lp, const double *ll,
+                                             int x, uint32_t iter, int i0, int k) {
     if (iter % (uint32_t)d.thin != 0) return;
-    const int k = (int)(iter / (uint32_t)d.thin);
-    if (k >= d.nmc) return;
+    const int col = (int)(iter / (uint32_t)d.thin);
+    if (col >= d.nmc || k <= 0) return;
     const int n = d.nchain, P = d.P;
-    double *ot = d.out_theta + ((size_t)x * d.nmc + k) * n * P;
-    for (int e = threadIdx.x; e < n * P; e += blockDim.x) ot[e] = th[e];
-    for (int c = threadIdx.x; c < n; c += blockDim.x) {
-        d.out_lp[((size_t)x * d.nmc + k) * n + c] = lp[c];
-        d.out_ll[((size_t)x * d.nmc + k) * n + c] = ll[c];
+    double *ot = d.out_theta + ((size_t)x * d.nmc + col) * n * P;
+    for (int e = threadIdx.x; e < k * P; e += blockDim.x) ot[(size_t)i0 * P + e] = th[(size_t)i0 * P + e];
+    for (int c = threadIdx.x; c < k; c += blockDim.x) {
+        d.out_lp[((size_t)x * d.nmc + col) * n + i0 + c] = lp[i0 + c];
+        d.out_ll[((size_t)x * d.nmc + col) * n + i0 + c] = ll[i0 + c];
     }
 }
 
@@ -1613,15 +1614,23 @@ __global__ __launch_bounds__(kBlock, 2) void subject_sweep_kernel(Dev d, SweepAr
         if (k > 0) proposal_stage1(d, job, L, st, lo, hi, i0, k, para, late);
     }
     __syncthreads();
-    /* deferred store: the first launch of iteration i stores iteration i - 1 */
-    if (a.store_prev && g_begin == 0 && c0 == 0 && t == 0 && iter > 1)
-        store_subject(d, L.th, L.lp, L.ll, x, iter - 1);
+    /* the subject's migration ring runs in this launch, in this item */
+    const bool mig_act = blocked && single ? a.sweep == 0 : true;
+    const bool mig_here = (late ? g_end == G : g_begin == 0) && c0 == 0 && t == 0 && mig_act;
+    /* deferred store: the first sweep's launches of iteration i store
+     * iteration i - 1 from the rows staged above. Each item stores the chains
+     * it updates, whose rows no other item writes, after its likelihood
+     * phase (before its accept changes them; the stores then complete while
+     * it waits for the population step); in a migration iteration no item
+     * but the ring's writes, and the ring's item stores every chain before
+     * it starts. */
+    const bool store_due = a.store_prev && t == 0 && iter > 1;
+    if (__builtin_expect(store_due && migrate && mig_here, 0)) store_chains(d, L.th, L.lp, L.ll, x, iter - 1, 0, n);
     uint32_t *ready = a.chained ? d.item_ready + ((size_t)x * G) * a.rmax : nullptr;
     uint32_t *staged = a.chained ? d.item_staged + ((size_t)x * G) * a.rmax : nullptr;
     /* this item has read the subject's rows: a later group may change them */
     if (a.chained && t == 0 && threadIdx.x == 0)
         __hip_atomic_store(staged + (size_t)g_begin * a.rmax + rng, iter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (a.chained && migrate && g_begin == 0 && c0 == 0 && t == 0) publish_item(ready, rng, iter);
     if (a.chained && !migrate && g_begin > 0) {
         /* the previous group's items of this subject first; then the rows
          * of the earlier groups' chains, which they changed */
@@ -1658,15 +1667,11 @@ __global__ __launch_bounds__(kBlock, 2) void subject_sweep_kernel(Dev d, SweepAr
     }
 
     if (__builtin_expect(migrate, 0)) {
-        const bool act = blocked && single ? a.sweep == 0 : true;
         /* with the population step in these launches, a hierarchical
          * subject migrates in the last group's launch, after every chain's
-         * population step (and, all groups in one launch, after the first
-         * item has stored the previous iteration) */
-        const bool here = late ? g_end == G : g_begin == 0;
-        if (c0 == 0 && t == 0 && act && here) {
+         * population step */
+        if (mig_here) {
             const int para = (blocked && !single) ? a.sweep : -1;
-            if (late && a.chained) wait_items(d, ready, 1, iter);
             if (late) wait_chains(d, job.phi_index, 0, n, iter, true);
             subject_migration<MODE>(d, L, job, x, sd, para, a.cmax,
                                     ntile == 1 && a.tstage > 0 ? tv : TrialView{d.rt, d.tcode, 0});
@@ -1688,6 +1693,7 @@ __global__ __launch_bounds__(kBlock, 2) void subject_sweep_kernel(Dev d, SweepAr
             __syncthreads();
             proposals_loglike<MODE>(d, L, tv, k, a.chain_batch, a.cmax, s0, s1, j_lo, nj, ntile > 1,
                                     d.chunk_off[x], i0, built);
+            if (store_due) store_chains(d, L.th, L.lp, L.ll, x, iter - 1, i0, k);
             if (threadIdx.x == 0) {
                 int na = 0;
                 for (int c = 0; c < k; ++c) na += L.active[c];
