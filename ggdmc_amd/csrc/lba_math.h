@@ -212,6 +212,16 @@ constexpr double kExpErr = 1e-15;   /* exp(-z^2/2) in FP64 arithmetic */
 constexpr double kCondTol = 1e-5;
 constexpr double kUlp1 = 1.1102230246251565e-16;
 
+/* x / d correctly rounded, given y = 1/d correctly rounded: q = x*y, then
+ * r = x - q*d exactly (one FMA) and RN(q + r*y) = RN(x/d) (Markstein's
+ * correction step) for finite x; x = +-inf takes x*y, which is x/d there.
+ * Three FP64 operations instead of a division sequence. */
+GGDMC_FN double div_rn(double x, double d, double y) {
+    const double q = x * y;
+    const double c = std::fma(std::fma(-q, d, x), y, q);
+    return std::isinf(x) ? q : c;
+}
+
 /* Winner: log(max(pdf, 1e-10)); NaN propagates. */
 template <class Ops>
 GGDMC_FN double mixed_winner_logpdf(const AccK &k, double rt, bool &ill) {
@@ -243,7 +253,7 @@ GGDMC_FN double mixed_loser_survivor(const AccK &k, double rt, bool &ill) {
      * quantises the reference's survivor, and it is reproduced exactly
      * whenever 1 - G is accurate to well under half an ulp of 1 */
     const double omG = (c.P2 - k.Pc) + (c.chi * c.dP - c.zs * c.dphi) * k.invA + k.Pc;
-    const double F = (G * k.inv_den < 0.5) ? G * k.inv_den : (1.0 - omG) / k.den;
+    const double F = (G * k.inv_den < 0.5) ? G * k.inv_den : div_rn(1.0 - omG, k.den, k.inv_den);
     const double S0 = 1.0 - (F > kFloor ? F : (F == F ? kFloor : F));
     /* error of F from the tails, plus the reference's own noise from Phi
      * values rounded near 1 (z > 0), against the survivor */

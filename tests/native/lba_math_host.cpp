@@ -29,6 +29,14 @@ struct HostOps {
 
 extern "C" {
 
+// x[i] / d[i] through the batched path's division from the reciprocal.
+void host_div_rn(const double *x, const double *d, int64_t n, double *out) {
+    for (int64_t i = 0; i < n; ++i) {
+        const double y = 1.0 / d[i];
+        out[i] = ggdmc::div_rn(x[i], d[i], y);
+    }
+}
+
 // One accumulator at one response time: the mixed-precision winner log density
 // and loser survivor, and the naive FP64 (reference formula) pdf and CDF.
 void host_acc_eval(const double *pars, double rt, double *out4) { /* out4: 6 values */

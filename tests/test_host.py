@@ -157,6 +157,32 @@ def test_ill_conditioned_evaluations_fall_back_to_the_reference_formula():
     assert flagged < 0.05 * n
 
 
+def test_division_from_reciprocal_is_correctly_rounded():
+    # the loser survivor's F = (1 - omG) / den divides through the correctly
+    # rounded reciprocal with one correction step: bit for bit IEEE division
+    # (numpy's) on x in the survivor range and den over [1e-10, 1], with
+    # significands of den near all ones and near powers of two
+    L = C.CDLL(_HOST)
+    d = C.POINTER(C.c_double)
+    L.host_div_rn.argtypes = [d, d, C.c_int64, d]
+    rng = np.random.default_rng(12)
+    n = 2_000_000
+    x = np.concatenate([rng.uniform(0.25, 1.5, n), 1.0 - np.ldexp(rng.random(n), -rng.integers(0, 53, n))])
+    den = np.concatenate([np.exp(-23.0 * rng.random(n)), 1.0 - np.ldexp(rng.random(n), -rng.integers(0, 60, n))])
+    mant = np.concatenate([(1 << 52) - 1 - rng.integers(0, 1024, n // 2), rng.integers(0, 1024, n // 2)])
+    expo = (1022 - rng.integers(0, 34, n)).astype(np.uint64)
+    edge = ((expo << np.uint64(52)) | mant.astype(np.uint64)).view(np.float64)
+    den[:n] = np.where(rng.random(n) < 0.5, den[:n], edge)
+    den = np.clip(den, 1e-10, 1.0)
+    x = np.concatenate([x, [np.inf, -np.inf, np.nan, 0.0, 1.0]])
+    den = np.concatenate([den, [0.5, 0.84, 1.0, 0.3, 1e-10]])
+    out = np.empty_like(x)
+    L.host_div_rn(x.ctypes.data_as(d), den.ctypes.data_as(d), len(x), out.ctypes.data_as(d))
+    ref = x / den
+    same = (out.view(np.uint64) == ref.view(np.uint64)) | (np.isnan(out) & np.isnan(ref))
+    assert same.all(), (x[~same][:5], den[~same][:5])
+
+
 def test_tiny_survivors_reproduce_the_reference_rounding():
     # a sharp loser long after its typical finishing time: the reference's
     # survivor 1 - F is quantised by FP64 rounding near F = 1; the batched
