@@ -49,10 +49,10 @@ W_ALG = {2: 130.0, 4: 255.0}
 # decomposition the autotuner picks there (c3: 2 chains x all trials per item,
 # c5: a whole chain group per item, both with every chain group in one
 # launch); the population workgroups of the launch are included.
-W_ISSUED = {"c3": 660.0, "c5": 824.0}
+W_ISSUED = {"c3": 664.0, "c5": 824.5}
 # HBM bytes per launch of the subject sweep kernel: rocprofv3 FETCH_SIZE x 2.0
 # + WRITE_SIZE x 1.0 (factors of a 256 MiB copy kernel; profiles/).
-TRAFFIC = {"c3": 2.678e6, "c5": 4.244e7}
+TRAFFIC = {"c3": 2.700e6, "c5": 4.264e7}
 # 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
 VALU_PEAK_TSLOTS = 256 * 4 * 16 * 2.4e9 / 1e12
 
