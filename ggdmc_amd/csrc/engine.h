@@ -35,6 +35,7 @@ constexpr int kWaves = kBlock / 64;
 constexpr int kMaxChains = 256;    /* chains per run */
 constexpr int kChunk = 128;        /* trials per canonical chunk sum */
 constexpr int kStatBlock = 32;     /* subjects per block of the population statistics */
+constexpr int kItersPerLaunch = 8; /* iterations per subject launch in launch mode 3 */
 
 struct JobDev {
     int32_t kind;           /* GGDMC_KIND_* */
@@ -125,6 +126,10 @@ struct SweepArgs {
                              item waits for the previous group's items of its subject */
     int32_t rmax;         /* chain ranges per group (item_ready stride) */
     int32_t tiles;        /* some subject's trials are split over several items */
+    int32_t iters;        /* consecutive iterations in this launch (chained only; else 1): the
+                             grid is iters segments of seg_wgs workgroups, segment s running
+                             iteration *iter_base + iter_off + s */
+    int32_t seg_wgs;      /* phi_wgs + n_items rounded up to a multiple of 8 (iters > 1) */
 };
 
 } // namespace ggdmc

@@ -138,6 +138,7 @@ struct ggdmc_session {
      * of its own kernel: */
     bool overlap = false;
     bool chained = false;         /* every chain group in one subject launch */
+    int ipl = 1;                  /* iterations per subject launch (chained only) */
     bool tiled = false;           /* some subject's trials split over several items */
     int rmax = 1;                 /* chain ranges per group (item_ready stride) */
     int ovl_stats_lds = 0;
@@ -169,9 +170,9 @@ struct ggdmc_session {
     }
 
     void build(const ggdmc_config *c, const ggdmc_model *model, const ggdmc_job *js, int nj);
-    void set_decomp(bool fused_new, int k_new, int tpt_new, bool chained_new = false);
+    void set_decomp(bool fused_new, int k_new, int tpt_new, bool chained_new = false, int ipl_new = 1);
     void default_decomp();
-    void enqueue_iteration(uint32_t off, std::vector<hipEvent_t> *ev = nullptr);
+    void enqueue_iteration(uint32_t off, std::vector<hipEvent_t> *ev = nullptr, int count = 1);
     void flush_store();
     void run_batched(uint32_t n_iter);
     void run_plain(uint32_t n_iter);
@@ -535,18 +536,19 @@ void ggdmc_session::build(const ggdmc_config *c, const ggdmc_model *model, const
  * fill the chip: chains per item so that subjects x chain ranges >= 1024. */
 void ggdmc_session::default_decomp() {
     const int ngmax = (d.nchain + d.G - 1) / d.G;
-    require(cfg.launch_mode >= 0 && cfg.launch_mode <= 2, "launch_mode must be 0, 1 or 2");
-    const bool one = cfg.launch_mode == 2;
+    require(cfg.launch_mode >= 0 && cfg.launch_mode <= 3, "launch_mode must be 0, 1, 2 or 3");
+    const bool one = cfg.launch_mode >= 2;
+    const int ipl_cfg = cfg.launch_mode == 3 ? kItersPerLaunch : 1;
     if (cfg.chains_per_item < 0) {
         require(cfg.trials_per_tile == 0, "one item per subject needs trials_per_tile = 0");
         require(!one, "one item per subject already covers every group in one launch");
         set_decomp(true, 0, 0);
     } else if (cfg.chains_per_item > 0 || cfg.trials_per_tile > 0) {
-        set_decomp(false, cfg.chains_per_item, cfg.trials_per_tile, one);
+        set_decomp(false, cfg.chains_per_item, cfg.trials_per_tile, one, ipl_cfg);
     } else {
         int k = ngmax;
         while (k > 1 && (int64_t)d.S * ((ngmax + k - 1) / k) < 1024) k = (k + 1) / 2;
-        set_decomp(false, k, 0, one);
+        set_decomp(false, k, 0, one, ipl_cfg);
     }
 }
 
@@ -554,15 +556,17 @@ void ggdmc_session::default_decomp() {
  * subject's chain ranges of k_new chains x trial tiles of tpt_new trials
  * (rounded up to a multiple of kChunk; 0 = whole subject); or, fused, one
  * item per subject that sweeps every group in one launch. Sizes the LDS. */
-void ggdmc_session::set_decomp(bool fused_new, int k_new, int tpt_new, bool chained_new) {
+void ggdmc_session::set_decomp(bool fused_new, int k_new, int tpt_new, bool chained_new, int ipl_new) {
     if (tpt_new > 0) tpt_new = ((tpt_new + kChunk - 1) / kChunk) * kChunk;
     require(!fused_new || tpt_new == 0, "one item per subject needs whole subjects");
     require(!(fused_new && chained_new), "one item per subject already covers every group");
     require(!chained_new || nsweep_sub == 1, "every group in one launch needs unblocked subject sweeps");
+    require(ipl_new == 1 || chained_new, "several iterations per launch need every group in one launch");
     const int n = d.nchain, G = d.G;
     const int ngmax = (n + G - 1) / G;
     fused = fused_new;
     chained = chained_new;
+    ipl = ipl_new;
     tiled = false;
     tpt = tpt_new;
     d.tpt = tpt;
@@ -675,6 +679,8 @@ void ggdmc_session::set_decomp(bool fused_new, int k_new, int tpt_new, bool chai
                             ggdmc_internal_phi_chain_lds(d.P, max_sub)});
         overlap = ovl_lds <= limit;
     }
+    require(ipl == 1 || d.n_phi == 0 || overlap,
+            "several iterations per launch need the population step inside the subject launch");
     drop_graph();
 }
 
@@ -683,7 +689,7 @@ void ggdmc_session::set_decomp(bool fused_new, int k_new, int tpt_new, bool chai
  * chain group), or per sweep with one item per subject over every group. The
  * first subject launch stores the previous iteration's subject states when
  * due. ev, if given, receives an event pair around every subject launch. */
-void ggdmc_session::enqueue_iteration(uint32_t off, std::vector<hipEvent_t> *ev) {
+void ggdmc_session::enqueue_iteration(uint32_t off, std::vector<hipEvent_t> *ev, int count) {
     if (d.n_phi > 0 && !overlap) HIPCHECK(ggdmc_internal_launch_phi(&d, off, phi_lds, phi_stats_lds, stream));
     if (d.S == 0) return;
     const int lk = mode == GGDMC_FP64_BATCHED ? GGDMC_FP64_BATCHED : GGDMC_MIXED_BATCHED;
@@ -695,9 +701,14 @@ void ggdmc_session::enqueue_iteration(uint32_t off, std::vector<hipEvent_t> *ev)
             const bool with_phi = overlap && sw == 0;
             /* chains of the launch: group g, or every group */
             const int nc = chained ? d.nchain : ((g + 1) * d.nchain) / d.G - (g * d.nchain) / d.G;
+            const int pw = with_phi ? d.n_phi * nc : 0;
+            /* several iterations (count) per launch: segments of a multiple of
+             * 8 workgroups, so that every segment maps items to XCDs alike */
+            const int multi = chained && ipl > 1 ? count : 1;
+            const int seg_wgs = multi > 1 ? (pw + list_n[g] + 7) / 8 * 8 : pw + list_n[g];
             SweepArgs a{off, sw, (fused || chained) ? 0 : g, (fused || chained) ? d.G : g + 1, (int)store,
                         chain_batch, kmax, cmax, tstage, list_n[g], lists[g],
-                        with_phi ? d.n_phi * nc : 0, ovl_stats_lds, (int)chained, rmax, (int)tiled};
+                        pw, ovl_stats_lds, (int)chained, rmax, (int)tiled, multi, seg_wgs};
             hipEvent_t e0 = nullptr, e1 = nullptr;
             if (ev) {
                 HIPCHECK(hipEventCreate(&e0));
@@ -726,7 +737,8 @@ void ggdmc_session::capture(int K) {
     hipGraph_t g = nullptr;
     HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     try {
-        for (int k = 0; k < K; ++k) enqueue_iteration((uint32_t)k);
+        const int step = chained ? ipl : 1;
+        for (int k = 0; k < K; k += step) enqueue_iteration((uint32_t)k, nullptr, std::min(step, K - k));
     } catch (...) {
         (void)hipStreamEndCapture(stream, &g);
         if (g) (void)hipGraphDestroy(g);
@@ -751,9 +763,10 @@ void ggdmc_session::run_batched(uint32_t n_iter) {
         } else {
             set_base_kernel<<<1, 1, 0, stream>>>(iter_base, next_iter);
             HIPCHECK(hipGetLastError());
-            enqueue_iteration(0);
-            next_iter += 1;
-            n_iter -= 1;
+            const int cnt = (int)std::min<uint32_t>(chained ? (uint32_t)ipl : 1u, n_iter);
+            enqueue_iteration(0, nullptr, cnt);
+            next_iter += (uint32_t)cnt;
+            n_iter -= (uint32_t)cnt;
         }
     }
     flush_store();
@@ -790,9 +803,10 @@ void ggdmc_session::autotune() {
         bool fused;
         int k, tpt;
         bool chained;
+        int ipl;
     };
     const int ngmax = (d.nchain + d.G - 1) / d.G;
-    std::vector<Cand> cands{{true, 0, 0, false}};
+    std::vector<Cand> cands{{true, 0, 0, false, 1}};
     std::vector<int> ks;
     for (int k : {1, 2, 4, 8, ngmax})
         if (k <= ngmax && std::find(ks.begin(), ks.end(), k) == ks.end()) ks.push_back(k);
@@ -802,8 +816,13 @@ void ggdmc_session::autotune() {
             for (int t : {0, 256, 1024})
                 if (t == 0 || t < max_ntr) {
                     const int64_t nt = t == 0 ? 1 : (max_ntr + t - 1) / t;
-                    if ((int64_t)d.S * ((ngmax + k - 1) / k) * nt <= ((int64_t)1 << 22))
-                        cands.push_back({false, k, t, one == 1});
+                    if ((int64_t)d.S * ((ngmax + k - 1) / k) * nt <= ((int64_t)1 << 22)) {
+                        cands.push_back({false, k, t, one == 1, 1});
+                        /* several iterations per launch: whole subjects, the
+                         * population step inside the launch */
+                        if (one == 1 && t == 0 && cfg.launch_mode != 2)
+                            cands.push_back({false, k, t, true, kItersPerLaunch});
+                    }
                 }
     const size_t nth = (size_t)d.S * d.nchain * d.P, nch = (size_t)d.S * d.nchain;
     const size_t nph = (size_t)d.n_phi * d.nchain * 2 * d.P, npc = (size_t)d.n_phi * d.nchain;
@@ -848,9 +867,14 @@ void ggdmc_session::autotune() {
     const uint32_t it0 = next_iter;
     float best = 1e30f;
     Cand pick = cands[0];
-    const int K = 4;
+    const int K = kItersPerLaunch;
     for (const Cand &c : cands) {
-        set_decomp(c.fused, c.k, c.tpt, c.chained);
+        try {
+            set_decomp(c.fused, c.k, c.tpt, c.chained, c.ipl);
+        } catch (const Error &) {
+            if (c.ipl == 1) throw;
+            continue;   /* no population step inside the launch for this design */
+        }
         next_iter = it0;
         capture(K);
         run_batched(K); /* warm-up */
@@ -868,7 +892,7 @@ void ggdmc_session::autotune() {
         }
     }
     next_iter = it0;
-    set_decomp(pick.fused, pick.k, pick.tpt, pick.chained);
+    set_decomp(pick.fused, pick.k, pick.tpt, pick.chained, pick.ipl);
 }
 
 static int fail(const std::exception &e) {
@@ -917,7 +941,7 @@ int ggdmc_session_create(const ggdmc_config *cfg, const ggdmc_model *model, cons
         s->build(cfg, model, jobs, njobs);
         if (s->mode != GGDMC_FP64_PLAIN) {
             /* tune the tile shape when the run is long enough to repay it */
-            if (cfg->trials_per_tile == 0 && cfg->chains_per_item == 0 && cfg->launch_mode != 2 &&
+            if (cfg->trials_per_tile == 0 && cfg->chains_per_item == 0 && cfg->launch_mode < 2 &&
                 s->last_iter >= 64)
                 s->autotune();
             const uint32_t total = s->last_iter;
@@ -963,7 +987,7 @@ int ggdmc_session_decomposition(ggdmc_session *s, int32_t *chains_per_item,
                                 int32_t *trials_per_tile, int32_t *items_per_launch,
                                 int32_t *launch_mode) {
     if (!s) return invalid("session is null");
-    if (launch_mode) *launch_mode = s->chained ? 2 : 1;
+    if (launch_mode) *launch_mode = s->ipl > 1 ? 3 : (s->chained ? 2 : 1);
     if (chains_per_item) *chains_per_item = s->fused ? -1 : s->kchain;
     if (trials_per_tile) *trials_per_tile = s->tpt;
     if (items_per_launch)
@@ -998,11 +1022,14 @@ int ggdmc_session_profile(ggdmc_session *s, int32_t n_iter, double *subject_kern
         const uint32_t left = s->next_iter <= s->last_iter ? s->last_iter - s->next_iter + 1 : 0;
         const uint32_t n = std::min<uint32_t>((uint32_t)n_iter, left);
         std::vector<hipEvent_t> ev;
-        for (uint32_t k = 0; k < n; ++k) {
+        /* launches as the run makes them (several iterations each in launch mode 3) */
+        const uint32_t step = s->chained ? (uint32_t)s->ipl : 1u;
+        for (uint32_t k = 0; k < n; k += step) {
+            const int cnt = (int)std::min(step, n - k);
             set_base_kernel<<<1, 1, 0, s->stream>>>(s->iter_base, s->next_iter);
             HIPCHECK(hipGetLastError());
-            s->enqueue_iteration(0, &ev);
-            s->next_iter += 1;
+            s->enqueue_iteration(0, &ev, cnt);
+            s->next_iter += (uint32_t)cnt;
         }
         s->flush_store();
         const size_t q = ev.size();

@@ -303,13 +303,21 @@ __device__ __forceinline__ SweepLds carve(char *base, int n, int P, int kmax, in
 /* dst[i] = src[i], start <= i < count, by the whole workgroup with four
  * loads in flight per thread (global -> LDS staging). Loads use clamped
  * indices, so no branch separates them. */
-template <class T>
+/* *p, or with AG a device-scope load: data written earlier in the same launch
+ * by a workgroup on another XCD (several iterations per launch). */
+template <bool AG, class T>
+__device__ __forceinline__ T ld(const T *p) {
+    if constexpr (AG) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else return *p;
+}
+
+template <class T, bool AG = false>
 __device__ __forceinline__ void stage_copy(T *dst, const T *src, int count, int start = 0) {
     const int bs = blockDim.x;
     for (int i0 = start + (int)threadIdx.x; i0 < count; i0 += 4 * bs) {
         T v[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = src[min(i0 + u * bs, count - 1)];
+        for (int u = 0; u < 4; ++u) v[u] = ld<AG>(src + min(i0 + u * bs, count - 1));
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int i = i0 + u * bs;
@@ -323,7 +331,7 @@ __device__ __forceinline__ void stage_copy(T *dst, const T *src, int count, int 
  * copies any elements beyond U * blockDim. Calling every piece's load()
  * before any store() keeps all of a thread's loads in flight together, so
  * the step costs one memory latency however many arrays it stages. */
-template <class T, int U>
+template <class T, int U, bool AG = false>
 struct Piece {
     T *dst;
     const T *src;
@@ -332,7 +340,7 @@ struct Piece {
     __device__ __forceinline__ void load() {
         if (count <= 0) return;
 #pragma unroll
-        for (int u = 0; u < U; ++u) v[u] = src[min((int)threadIdx.x + u * (int)blockDim.x, count - 1)];
+        for (int u = 0; u < U; ++u) v[u] = ld<AG>(src + min((int)threadIdx.x + u * (int)blockDim.x, count - 1));
     }
     __device__ __forceinline__ void store() {
         if (count <= 0) return;
@@ -341,7 +349,7 @@ struct Piece {
             const int i = (int)threadIdx.x + u * (int)blockDim.x;
             if (i < count) dst[i] = v[u];
         }
-        if (count > U * (int)blockDim.x) stage_copy(dst, src, count, U * (int)blockDim.x);
+        if (count > U * (int)blockDim.x) stage_copy<T, AG>(dst, src, count, U * (int)blockDim.x);
     }
 };
 
@@ -534,13 +542,14 @@ __device__ __forceinline__ void accept_proposals(const Dev &d, const SweepLds &L
         if (a) {
             L.lp[i] = L.tlp[c];
             L.ll[i] = L.tll[c];
-            d.lp[(size_t)x * n + i] = L.tlp[c];
-            d.ll[(size_t)x * n + i] = L.tll[c];
+            __hip_atomic_store(d.lp + (size_t)x * n + i, L.tlp[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(d.ll + (size_t)x * n + i, L.tll[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
     __syncthreads();
-    /* device-scope stores: a later group's item of this subject may read the
-     * rows in the same launch */
+    /* device-scope stores: a later group's item of this subject (or, with
+     * several iterations per launch, a later iteration's) may read the rows
+     * in the same launch */
     for (int e = threadIdx.x; e < k * P; e += blockDim.x) {
         const int c = e / P;
         if (!L.acc[c]) continue;
@@ -595,15 +604,16 @@ __device__ __forceinline__ void subject_migration(const Dev &d, const SweepLds &
             if (a) {
                 L.lp[dst] = L.tlp[0];
                 L.ll[dst] = L.tll[0];
-                d.lp[(size_t)x * n + dst] = L.tlp[0];
-                d.ll[(size_t)x * n + dst] = L.tll[0];
+                __hip_atomic_store(d.lp + (size_t)x * n + dst, L.tlp[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(d.ll + (size_t)x * n + dst, L.tll[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
         __syncthreads();
         if (L.flag[1]) {
             for (int p = threadIdx.x; p < P; p += blockDim.x) {
                 L.th[(size_t)dst * P + p] = L.prop[p];
-                d.theta[((size_t)x * n + dst) * P + p] = L.prop[p];
+                __hip_atomic_store(d.theta + ((size_t)x * n + dst) * P + p, L.prop[p], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
             }
         }
         __syncthreads();
@@ -809,9 +819,10 @@ __device__ double hyper_logprior(const Dev &d, const JobDev &job, const double *
  * e = q * nblk + j in [e0, e1), one thread each, into bst[e][4]. */
 /* Loads the 1 <= nb <= kStatBlock values src[i * stride] into registers
  * (clamped indices: the loads issue back to back). */
+template <bool AG = false>
 __device__ __forceinline__ void block_load(const double *src, size_t stride, int nb, double (&v)[kStatBlock]) {
 #pragma unroll
-    for (int i = 0; i < kStatBlock; ++i) v[i] = src[(size_t)min(i, nb - 1) * stride];
+    for (int i = 0; i < kStatBlock; ++i) v[i] = ld<AG>(src + (size_t)min(i, nb - 1) * stride);
 }
 
 /* Sum, M2 about the mean sum / nb, min, max and a NaN flag of v[0..nb), in
@@ -1232,6 +1243,7 @@ __host__ __device__ inline size_t phi_chain_lds_bytes(int P, int nblk) {
 __device__ __forceinline__ void wait_chains(const Dev &d, int f, int c_lo, int c_n, uint32_t iter,
                                             bool acquire);
 
+template <bool AG>
 __device__ __forceinline__ void phi_chain(const Dev &d, int f, int i, uint32_t iter, char *lds_raw,
                                           bool chained) {
     const int n = d.nchain, P = d.P, P2 = 2 * P, G = d.G;
@@ -1274,10 +1286,10 @@ __device__ __forceinline__ void phi_chain(const Dev &d, int f, int i, uint32_t i
     /* loads in one batch: the subject values of this chain (one block),
      * rows i, m0, m1, chain i's cached logarithms, the prior constants */
     double v[kStatBlock];
-    if (nblk == 1) block_load(tbase + min(t, P - 1), nq, S, v);
+    if (nblk == 1) block_load<AG>(tbase + min(t, P - 1), nq, S, v);
     {
-        Piece<double, 1> pxi{xi, d.phi + (off + i) * P2, P2};
-        Piece<double, 1> ppc{pc, d.pcache + (off + i) * P2, P2};
+        Piece<double, 1, AG> pxi{xi, d.phi + (off + i) * P2, P2};
+        Piece<double, 1, AG> ppc{pc, d.pcache + (off + i) * P2, P2};
         const int qq = min(t, P2 - 1), pq = min(t, P - 1);
         const int k = job.hprior_off + qq, kp = job.prior_off + pq;
         pxi.load();
@@ -1290,8 +1302,8 @@ __device__ __forceinline__ void phi_chain(const Dev &d, int f, int i, uint32_t i
         const double h4 = d.scache[2 * k], h5 = d.scache[2 * k + 1];
         const int hd = d.pr_dist[k];
         const double b0 = d.pr_lo[kp], b1 = d.pr_up[kp];
-        const double lp0 = d.phi_lp[off + i];
-        const int valid = d.pcache_valid[off + i];
+        const double lp0 = ld<AG>(d.phi_lp + off + i);
+        const int valid = ld<AG>(d.pcache_valid + off + i);
         pxi.store();
         ppc.store();
         if (t < P2) {
@@ -1340,7 +1352,7 @@ __device__ __forceinline__ void phi_chain(const Dev &d, int f, int i, uint32_t i
     if (nblk == 1) {
         if (t < P) direct_stats(v, S, stt + (size_t)t * 4);
         for (int r = t + bs; r < P; r += bs) {
-            block_load(tbase + r, nq, S, v);
+            block_load<AG>(tbase + r, nq, S, v);
             direct_stats(v, S, stt + (size_t)r * 4);
         }
     } else {
@@ -1349,7 +1361,7 @@ __device__ __forceinline__ void phi_chain(const Dev &d, int f, int i, uint32_t i
             const int nb = min(kStatBlock, S - j * kStatBlock);
             double sum, m2, mn, mx;
             bool nan;
-            block_load(tbase + (size_t)j * kStatBlock * nq + pr, nq, nb, v);
+            block_load<AG>(tbase + (size_t)j * kStatBlock * nq + pr, nq, nb, v);
             block_moments(v, nb, sum, m2, mn, mx, nan);
             double *o = bst + (size_t)e * 4;
             o[0] = nan ? __builtin_nan("") : sum;
@@ -1434,12 +1446,17 @@ __device__ __forceinline__ void phi_chain(const Dev &d, int f, int i, uint32_t i
  * hundreds of workgroups do it). */
 constexpr uint32_t kSpinLimit = 1u << 18;
 
+/* A flag holds the last iteration whose step it marks: iteration iter's step
+ * is done once flag >= iter (iterations only grow within a session; with
+ * several iterations in a launch a flag may already hold a later one). */
+__device__ __forceinline__ bool reached(uint32_t v, uint32_t iter) { return (int32_t)(v - iter) >= 0; }
+
 __device__ __forceinline__ void wait_chains(const Dev &d, int f, int c_lo, int c_n, uint32_t iter,
                                             bool acquire) {
     for (int c = threadIdx.x; c < c_n; c += blockDim.x) {
         const uint32_t *flag = d.phi_ready + (size_t)f * d.nchain + c_lo + c;
         uint32_t spins = 0;
-        while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != iter) {
+        while (!reached(__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), iter)) {
             /* after one failure every later wait gives up at once */
             if (++spins == kSpinLimit ||
                 __hip_atomic_load(d.fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
@@ -1462,17 +1479,63 @@ __device__ __forceinline__ void publish_item(uint32_t *ready, int r, uint32_t it
     if (threadIdx.x == 0) __hip_atomic_store(ready + r, iter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-/* Waits (bounded, as wait_chains) until ready[0..nr) all equal iter. */
+/* Waits (bounded, as wait_chains) until ready[0..nr) all reach iter. */
 __device__ __forceinline__ void wait_items(const Dev &d, const uint32_t *ready, int nr, uint32_t iter) {
     for (int r = threadIdx.x; r < nr; r += blockDim.x) {
         uint32_t spins = 0;
-        while (__hip_atomic_load(ready + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != iter) {
+        while (!reached(__hip_atomic_load(ready + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), iter)) {
             if (++spins == kSpinLimit ||
                 __hip_atomic_load(d.fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
                 atomicOr(d.fault, 1u);
                 break;
             }
             __builtin_amdgcn_s_sleep(4);
+        }
+    }
+    __syncthreads();
+}
+
+/* Bounded wait (as wait_chains) for one flag; false on giving up. */
+__device__ __forceinline__ void wait_flag(const Dev &d, const uint32_t *flag, uint32_t iter) {
+    uint32_t spins = 0;
+    while (!reached(__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), iter)) {
+        if (++spins == kSpinLimit || __hip_atomic_load(d.fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+            atomicOr(d.fault, 1u);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(4);
+    }
+}
+
+/* Several iterations per launch: every item of subject x (each group's
+ * chain ranges) has finished iteration iter. Items published with
+ * device-scope stores; the waiter reads the subject's state with
+ * device-scope loads. */
+__device__ __forceinline__ void wait_subject(const Dev &d, const SweepArgs &a, int x, uint32_t iter) {
+    const int n = d.nchain, G = d.G;
+    const uint32_t *ready = d.item_ready + (size_t)x * G * a.rmax;
+    for (int e = threadIdx.x; e < G * a.rmax; e += blockDim.x) {
+        const int g = e / a.rmax, r = e - g * a.rmax;
+        const int ng = group_lo(g + 1, n, G) - group_lo(g, n, G);
+        if (r < (ng + a.kmax - 1) / a.kmax) wait_flag(d, ready + e, iter);
+    }
+    __syncthreads();
+}
+
+/* Several iterations per launch: population state f's whole run has finished
+ * iteration iter (every chain's population step and, for a hierarchical run,
+ * every item of every subject). */
+__device__ __forceinline__ void wait_run(const Dev &d, const SweepArgs &a, int f, uint32_t iter) {
+    const int n = d.nchain, G = d.G;
+    const JobDev job = d.phi_jobs[f];
+    for (int c = threadIdx.x; c < n; c += blockDim.x) wait_flag(d, d.phi_ready + (size_t)f * n + c, iter);
+    if (job.kind == GGDMC_KIND_HIER) {
+        const int per = G * a.rmax;
+        const uint32_t *ready = d.item_ready + (size_t)job.first_subject * per;
+        for (int e = threadIdx.x; e < job.nsubject * per; e += blockDim.x) {
+            const int q = e % per, g = q / a.rmax, r = q - g * a.rmax;
+            const int ng = group_lo(g + 1, n, G) - group_lo(g, n, G);
+            if (r < (ng + a.kmax - 1) / a.kmax) wait_flag(d, ready + e, iter);
         }
     }
     __syncthreads();
@@ -1507,12 +1570,22 @@ __device__ __forceinline__ void late_prior(const Dev &d, const JobDev &job, cons
 }
 
 /* ------------------------------------------------------------ sweep kernel */
-template <int MODE>
+template <int MODE, bool MULTI>
 __global__ __launch_bounds__(kBlock, 2) void subject_sweep_kernel(Dev d, SweepArgs a) {
     extern __shared__ __attribute__((aligned(16))) char lds_raw[];
+    /* several iterations per launch (MULTI): segment seg of seg_wgs
+     * workgroups runs iteration base + seg; its workgroups wait only on
+     * lower workgroup ids (the previous segment, or earlier in their own),
+     * which in-order dispatch has started before them */
+    int seg = 0, wid = (int)blockIdx.x;
+    if constexpr (MULTI) {
+        seg = (int)blockIdx.x / a.seg_wgs;
+        wid = (int)blockIdx.x - seg * a.seg_wgs;
+        if (wid >= a.phi_wgs + a.n_items) return;   /* padding to a multiple of 8 */
+    }
     /* (the population workgroups are few: their code is laid out after the
      * subjects' path) */
-    if (__builtin_expect((int)blockIdx.x < a.phi_wgs, 0)) {
+    if (__builtin_expect(wid < a.phi_wgs, 0)) {
         /* the population step of the launch's chains, concurrent with the
          * subjects' proposals and likelihoods (they depend on it only through
          * the priors of the hierarchical subjects' proposals): one workgroup
@@ -1521,13 +1594,22 @@ __global__ __launch_bounds__(kBlock, 2) void subject_sweep_kernel(Dev d, SweepAr
         const int gb = a.chained ? 0 : a.g_begin, ge = a.chained ? d.G : a.g_end;
         const int c_lo = group_lo(gb, d.nchain, d.G);
         const int nc = group_lo(ge, d.nchain, d.G) - c_lo;
-        const int f = (int)blockIdx.x / nc, i = c_lo + (int)blockIdx.x % nc;
-        const uint32_t it = *d.iter_base + a.iter_off;
-        if (!phi_full(d, d.phi_jobs[f], it)) phi_chain(d, f, i, it, lds_raw, a.chained != 0);
-        else if (gb == 0 && i == 0) phi_block(d, it, 1, a.phi_stats_lds, f, lds_raw, true);
+        const int f = wid / nc, i = c_lo + wid % nc;
+        const uint32_t it = *d.iter_base + a.iter_off + (uint32_t)seg;
+        if (!phi_full(d, d.phi_jobs[f], it)) {
+            if (MULTI && seg > 0) wait_run(d, a, f, it - 1);
+            phi_chain<MULTI>(d, f, i, it, lds_raw, a.chained != 0);
+        } else if (gb == 0 && i == 0) {
+            if (MULTI && seg > 0) {
+                /* acquire: the whole-run step reads the state plainly */
+                wait_run(d, a, f, it - 1);
+                __threadfence();
+            }
+            phi_block(d, it, 1, a.phi_stats_lds, f, lds_raw, true);
+        }
         return;
     }
-    const int ib = (int)blockIdx.x - a.phi_wgs;
+    const int ib = wid - a.phi_wgs;
     const int4 item = a.items[ib];
     const int x = item.x;
     const int c0 = item.y >> 16, c1 = item.y & 0xFFFF;
@@ -1544,6 +1626,8 @@ __global__ __launch_bounds__(kBlock, 2) void subject_sweep_kernel(Dev d, SweepAr
     const int64_t t_hi = ntile > 1 ? min(s1, t_lo + (int64_t)d.tpt) : s1;
     const int nst = a.tstage > 0 ? (int)(t_hi - t_lo) : 0;
     const JobDev job = d.subj_jobs[x];
+    /* the subject's previous iteration, run earlier in this launch */
+    if (MULTI && seg > 0) wait_subject(d, a, x, *d.iter_base + a.iter_off + (uint32_t)seg - 1u);
     {
         /* loads first (they depend on the item only), all in flight together,
          * then the draws: exp table, state, trials, design rows, the subject
@@ -1555,9 +1639,9 @@ __global__ __launch_bounds__(kBlock, 2) void subject_sweep_kernel(Dev d, SweepAr
         const int tp = (int)threadIdx.x, pq = min(tp, P - 1);
         const double blo = d.pr_lo[job.prior_off + pq], bup = d.pr_up[job.prior_off + pq];
         const int pdi = job.kind == GGDMC_KIND_HIER ? GGDMC_DIST_TNORM : d.pr_dist[job.prior_off + pq];
-        Piece<double, 4> pth{L.th, d.theta + (size_t)x * n * P, n * P};
-        Piece<double, 1> plp{L.lp, d.lp + (size_t)x * n, n};
-        Piece<double, 1> pll{L.ll, d.ll + (size_t)x * n, n};
+        Piece<double, 4, MULTI> pth{L.th, d.theta + (size_t)x * n * P, n * P};
+        Piece<double, 1, MULTI> plp{L.lp, d.lp + (size_t)x * n, n};
+        Piece<double, 1, MULTI> pll{L.ll, d.ll + (size_t)x * n, n};
         Piece<double, 4> prt{L.trt, d.rt + t_lo, nst};
         Piece<uint16_t, 4> pcd{L.tcd, d.tcode + t_lo, nst};
         pex.load();
@@ -1588,7 +1672,7 @@ __global__ __launch_bounds__(kBlock, 2) void subject_sweep_kernel(Dev d, SweepAr
         }
     }
     const TrialView tv = nst > 0 ? TrialView{L.trt, L.tcd, t_lo} : TrialView{d.rt, d.tcode, 0};
-    const uint32_t iter = *d.iter_base + a.iter_off;
+    const uint32_t iter = *d.iter_base + a.iter_off + (uint32_t)seg;
     const bool single = job.kind != GGDMC_KIND_HIER;
     const bool blocked = d.blocked_sub != 0;
     const bool late = a.phi_wgs > 0 && !single;
@@ -1647,7 +1731,7 @@ __global__ __launch_bounds__(kBlock, 2) void subject_sweep_kernel(Dev d, SweepAr
                 const int m = (e & 1) ? L.m1[e >> 1] : L.m0[e >> 1];
                 const uint32_t *f = ready + (m - glo) / a.kmax;
                 uint32_t spins = 0;
-                while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != iter) {
+                while (!reached(__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), iter)) {
                     if (++spins == kSpinLimit ||
                         __hip_atomic_load(d.fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
                         atomicOr(d.fault, 1u);
@@ -1675,6 +1759,18 @@ __global__ __launch_bounds__(kBlock, 2) void subject_sweep_kernel(Dev d, SweepAr
             if (late) wait_chains(d, job.phi_index, 0, n, iter, true);
             subject_migration<MODE>(d, L, job, x, sd, para, a.cmax,
                                     ntile == 1 && a.tstage > 0 ? tv : TrialView{d.rt, d.tcode, 0});
+            /* the ring changed chains of every group: it marks every range done
+             * (no other item of a migrating subject publishes) */
+            if (a.chained) {
+                __builtin_amdgcn_s_waitcnt(0);
+                __syncthreads();
+                for (int e = threadIdx.x; e < G * a.rmax; e += blockDim.x) {
+                    const int g = e / a.rmax, r = e - g * a.rmax;
+                    const int ng = group_lo(g + 1, n, G) - group_lo(g, n, G);
+                    if (r < (ng + a.kmax - 1) / a.kmax)
+                        __hip_atomic_store(ready + e, iter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
         }
     } else {
         const int j_lo = (int)((t_lo - s0) / kChunk);
@@ -1711,7 +1807,7 @@ __global__ __launch_bounds__(kBlock, 2) void subject_sweep_kernel(Dev d, SweepAr
                 __syncthreads();
                 if (!L.flag[3]) continue;
                 __threadfence();
-                if (threadIdx.x == 0) *ctr = 0u;
+                if (threadIdx.x == 0) __hip_atomic_store(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 const int64_t cb = d.chunk_off[x], ncb = d.chunk_off[x + 1] - cb;
                 for (int c = threadIdx.x; c < k; c += blockDim.x) {
                     double s = 0.0;
@@ -2028,11 +2124,20 @@ size_t ggdmc_internal_phi_chain_lds(int P, int max_subjects) {
 
 hipError_t ggdmc_internal_launch_sweep(const Dev *d, const SweepArgs *a, int mode, size_t lds,
                                        hipStream_t stream) {
-    const dim3 grid(a->phi_wgs + a->n_items), block(kBlock);
+    const dim3 block(kBlock);
+    if (a->iters > 1) {
+        const dim3 grid(a->iters * a->seg_wgs);
+        if (mode == GGDMC_MIXED_BATCHED)
+            subject_sweep_kernel<MODE_MIXED, true><<<grid, block, lds, stream>>>(*d, *a);
+        else
+            subject_sweep_kernel<MODE_NAIVE, true><<<grid, block, lds, stream>>>(*d, *a);
+        return hipGetLastError();
+    }
+    const dim3 grid(a->phi_wgs + a->n_items);
     if (mode == GGDMC_MIXED_BATCHED)
-        subject_sweep_kernel<MODE_MIXED><<<grid, block, lds, stream>>>(*d, *a);
+        subject_sweep_kernel<MODE_MIXED, false><<<grid, block, lds, stream>>>(*d, *a);
     else
-        subject_sweep_kernel<MODE_NAIVE><<<grid, block, lds, stream>>>(*d, *a);
+        subject_sweep_kernel<MODE_NAIVE, false><<<grid, block, lds, stream>>>(*d, *a);
     return hipGetLastError();
 }
 

@@ -176,9 +176,11 @@ def test_decomposition_never_changes_results(gpu, precision):
     prec = getattr(gpu, precision)
     ref = None
     # (chains per item, trials per item, launch mode: 1 = a launch per chain
-    # group, 2 = every group in one launch, later groups waiting for earlier)
+    # group, 2 = every group in one launch, later groups waiting for earlier,
+    # 3 = as 2 with several iterations per launch)
     for k, tpt, lm in ((-1, 0, 1), (0, 0, 1), (1, 0, 1), (4, 128, 1), (2, 256, 1), (0, 384, 1),
-                       (0, 0, 2), (1, 0, 2), (4, 128, 2), (2, 256, 2)):
+                       (0, 0, 2), (1, 0, 2), (4, 128, 2), (2, 256, 2), (0, 0, 3), (1, 0, 3),
+                       (2, 0, 3), (4, 128, 3)):
         cfg = gpu.make_config(nmc=6, nchain=n, pop_migration_prob=0.2, sub_migration_prob=0.3,
                               precision=prec, sweep_groups=2, chains_per_item=k,
                               trials_per_tile=tpt, launch_mode=lm)
