@@ -44,15 +44,17 @@ METRIC = ("DE-MCMC iter/s + trial-LL evals/s: batched FP32 vs plain FP64 GPU, "
 # 170 + 21 with 4.
 W_ALG = {2: 130.0, 4: 255.0}
 # VALU lane-slots the subject sweep kernel actually issues per evaluation,
-# everything included (precision work, indexing, staging, sums, proposals):
-# rocprofv3 SQ_INSTS_VALU x 64 / evaluations of a launch (profiles/), at the
-# decomposition the autotuner picks there (c3: 2 chains x all trials per item,
-# c5: a whole chain group per item, both with every chain group in one
-# launch); the population workgroups of the launch are included.
-W_ISSUED = {"c3": 664.0, "c5": 824.5}
-# HBM bytes per launch of the subject sweep kernel: rocprofv3 FETCH_SIZE x 2.0
-# + WRITE_SIZE x 1.0 (factors of a 256 MiB copy kernel; profiles/).
-TRAFFIC = {"c3": 2.700e6, "c5": 4.264e7}
+# everything included (precision work, indexing, staging, sums, proposals,
+# the population workgroups, and the polling of items that wait for their
+# subject's previous iteration): rocprofv3 SQ_INSTS_VALU x 64 / evaluations of
+# a launch (profiles/), at the decomposition the autotuner picks there (c3: 2
+# chains x all trials per item, c5: a whole chain group per item; every chain
+# group and 32 iterations in one launch).
+W_ISSUED = {"c3": 775.0, "c5": 841.5}
+# HBM bytes per iteration of the subject sweep kernel at that decomposition:
+# rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors of a 256 MiB copy
+# kernel; profiles/) per launch, divided by the 32 iterations of a launch.
+TRAFFIC = {"c3": 1.007e6, "c5": 2.247e7}
 # 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
 VALU_PEAK_TSLOTS = 256 * 4 * 16 * 2.4e9 / 1e12
 
@@ -72,7 +74,7 @@ def parse():
     ap.add_argument("--cpu-cores", type=int, default=0,
                     help="threads for the aggregate CPU baseline (0: the host cores, at most 16)")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--profile-iters", type=int, default=20)
+    ap.add_argument("--profile-iters", type=int, default=32)
     ap.add_argument("--decomp", default="",
                     help="fixed work items 'k,t[,m]' (chains per item or -1, trials per item or 0, "
                          "launch mode 1 = a subject launch per chain group, 2 = one for every "
@@ -219,10 +221,12 @@ def main():
     k_ms, n_launch = sess.profile(a.profile_iters)
     e3 = sess.trial_evals()
     evals_per_launch = (e3 - e2) / max(n_launch, 1)
+    iters_per_launch = a.profile_iters / max(n_launch, 1)
     achieved = evals_per_launch * W_ALG[n_acc] / (k_ms / 1e3) / 1e12
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TSLOTS, 2),
                 "unit": "Tslot/s", "frac": round(achieved / VALU_PEAK_TSLOTS, 4),
-                "traffic": TRAFFIC.get(w.name), "kernel": "subject_sweep_kernel",
+                "traffic": TRAFFIC[w.name] * iters_per_launch if w.name in TRAFFIC else None,
+                "kernel": "subject_sweep_kernel", "iterations_per_launch": iters_per_launch,
                 "kernel_ms": round(k_ms, 4), "evals_per_launch": evals_per_launch,
                 "slots_per_eval": W_ALG[n_acc]}
     wi = W_ISSUED.get(w.name)

@@ -35,7 +35,7 @@ constexpr int kWaves = kBlock / 64;
 constexpr int kMaxChains = 256;    /* chains per run */
 constexpr int kChunk = 128;        /* trials per canonical chunk sum */
 constexpr int kStatBlock = 32;     /* subjects per block of the population statistics */
-constexpr int kItersPerLaunch = 8; /* iterations per subject launch in launch mode 3 */
+constexpr int kItersPerLaunch = 32; /* iterations per subject launch in launch mode 3 */
 
 struct JobDev {
     int32_t kind;           /* GGDMC_KIND_* */

@@ -867,7 +867,7 @@ void ggdmc_session::autotune() {
     const uint32_t it0 = next_iter;
     float best = 1e30f;
     Cand pick = cands[0];
-    const int K = kItersPerLaunch;
+    const int K = 8;   /* iterations per timed replay (one launch in launch mode 3) */
     for (const Cand &c : cands) {
         try {
             set_decomp(c.fused, c.k, c.tpt, c.chained, c.ipl);

@@ -68,7 +68,7 @@ typedef struct {
     int32_t launch_mode;    /* subject launches per sweep: 0 = autotuned with the two above
                                (else 1), 1 = one per chain group, 2 = one for every chain
                                group (an item waits for the previous group's items of its
-                               subject), 3 = as 2 with 8 iterations per launch (an item
+                               subject), 3 = as 2 with 32 iterations per launch (an item
                                also waits for its subject's previous iteration); never
                                changes results */
     uint64_t iter_offset;   /* Philox iteration offset (restart continues a stream) */
