@@ -1626,12 +1626,12 @@ __global__ __launch_bounds__(kBlock, 2) void subject_sweep_kernel(Dev d, SweepAr
     const int64_t t_hi = ntile > 1 ? min(s1, t_lo + (int64_t)d.tpt) : s1;
     const int nst = a.tstage > 0 ? (int)(t_hi - t_lo) : 0;
     const JobDev job = d.subj_jobs[x];
-    /* the subject's previous iteration, run earlier in this launch */
-    if (MULTI && seg > 0) wait_subject(d, a, x, *d.iter_base + a.iter_off + (uint32_t)seg - 1u);
     {
         /* loads first (they depend on the item only), all in flight together,
          * then the draws: exp table, state, trials, design rows, the subject
-         * prior's bounds and families */
+         * prior's bounds and families (several iterations per launch: the
+         * state after the draws, once the subject's previous iteration, run
+         * earlier in this launch, is done) */
         const int per = d.n_cond * d.n_acc * 6;
         Piece<double, 1> pex{g_exp2tab, d.exp2tab, 64};
         Piece<int32_t, 1> psr{L.psrc, d.par_src, per};
@@ -1639,23 +1639,27 @@ __global__ __launch_bounds__(kBlock, 2) void subject_sweep_kernel(Dev d, SweepAr
         const int tp = (int)threadIdx.x, pq = min(tp, P - 1);
         const double blo = d.pr_lo[job.prior_off + pq], bup = d.pr_up[job.prior_off + pq];
         const int pdi = job.kind == GGDMC_KIND_HIER ? GGDMC_DIST_TNORM : d.pr_dist[job.prior_off + pq];
-        Piece<double, 4, MULTI> pth{L.th, d.theta + (size_t)x * n * P, n * P};
-        Piece<double, 1, MULTI> plp{L.lp, d.lp + (size_t)x * n, n};
-        Piece<double, 1, MULTI> pll{L.ll, d.ll + (size_t)x * n, n};
+        [[maybe_unused]] Piece<double, 4> pth{L.th, d.theta + (size_t)x * n * P, n * P};
+        [[maybe_unused]] Piece<double, 1> plp{L.lp, d.lp + (size_t)x * n, n};
+        [[maybe_unused]] Piece<double, 1> pll{L.ll, d.ll + (size_t)x * n, n};
         Piece<double, 4> prt{L.trt, d.rt + t_lo, nst};
         Piece<uint16_t, 4> pcd{L.tcd, d.tcode + t_lo, nst};
         pex.load();
-        pth.load();
-        plp.load();
-        pll.load();
+        if constexpr (!MULTI) {
+            pth.load();
+            plp.load();
+            pll.load();
+        }
         prt.load();
         pcd.load();
         psr.load();
         pcs.load();
         pex.store();
-        pth.store();
-        plp.store();
-        pll.store();
+        if constexpr (!MULTI) {
+            pth.store();
+            plp.store();
+            pll.store();
+        }
         prt.store();
         pcd.store();
         psr.store();
@@ -1692,10 +1696,24 @@ __global__ __launch_bounds__(kBlock, 2) void subject_sweep_kernel(Dev d, SweepAr
         return min(c1 == kAllChains ? hi - lo : c1, hi - lo) - c0;
     };
     if (!migrate) {
-        /* stage 1 of the first group overlaps the state loads */
+        /* stage 1 of the first group: partners and draws, no state */
         int lo, hi, i0;
         const int k = group_k(g_begin, lo, hi, i0);
         if (k > 0) proposal_stage1(d, job, L, st, lo, hi, i0, k, para, late);
+    }
+    if constexpr (MULTI) {
+        /* the state, after the subject's previous iteration (earlier in this
+         * launch: device-scope loads) */
+        if (seg > 0) wait_subject(d, a, x, iter - 1u);
+        Piece<double, 4, true> pth{L.th, d.theta + (size_t)x * n * P, n * P};
+        Piece<double, 1, true> plp{L.lp, d.lp + (size_t)x * n, n};
+        Piece<double, 1, true> pll{L.ll, d.ll + (size_t)x * n, n};
+        pth.load();
+        plp.load();
+        pll.load();
+        pth.store();
+        plp.store();
+        pll.store();
     }
     __syncthreads();
     /* the subject's migration ring runs in this launch, in this item */
