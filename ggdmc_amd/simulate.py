@@ -1,6 +1,6 @@
 """Synthetic LBA data for the benchmark workloads.
 
-Generation rule (the BASELINE data rule, as in lbaModel::simulate used by
+Generation rule (the LBA simulation rule, as in lbaModel::simulate used by
 tests/testthat/Group1/data/*_lba_*.r): per trial and accumulator j, start
 a_j ~ U(0, A_j), threshold b_j = A_j + B_j, drift v_j ~ N(mean_v_j, sd_v_j)
 with all drifts redrawn when none is positive; RT = t0 + min over positive
