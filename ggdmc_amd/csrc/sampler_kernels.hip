@@ -1257,10 +1257,6 @@ __device__ __forceinline__ void phi_chain(const Dev &d, int f, int i, uint32_t i
     int m0, m1;
     double U;
     partners(st, i, group_lo(g, n, G), group_lo(g + 1, n, G), n, m0, m1, U);
-    /* every chain group in one launch: the earlier groups' chains first (the
-     * partners among them are read after their step, with device-scope
-     * loads; the later groups' chains wait for this one before they change) */
-    if (chained && g > 0) wait_chains(d, f, 0, group_lo(g, n, G), iter, false);
 
     PhiLds L{};   /* the term helpers read hc, hd, sb */
     double *p = reinterpret_cast<double *>(lds_raw);
@@ -1294,10 +1290,6 @@ __device__ __forceinline__ void phi_chain(const Dev &d, int f, int i, uint32_t i
         const int k = job.hprior_off + qq, kp = job.prior_off + pq;
         pxi.load();
         ppc.load();
-        const int qp = min(t, P2 - 1);
-        const double *r0 = d.phi + (off + m0) * P2, *r1 = d.phi + (off + m1) * P2;
-        const double v0 = chained ? __hip_atomic_load(r0 + qp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : r0[qp];
-        const double v1 = chained ? __hip_atomic_load(r1 + qp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : r1[qp];
         const double h0 = d.pr_p0[k], h1 = d.pr_p1[k], h2 = d.pr_lo[k], h3 = d.pr_up[k];
         const double h4 = d.scache[2 * k], h5 = d.scache[2 * k + 1];
         const int hd = d.pr_dist[k];
@@ -1306,14 +1298,6 @@ __device__ __forceinline__ void phi_chain(const Dev &d, int f, int i, uint32_t i
         const int valid = ld<AG>(d.pcache_valid + off + i);
         pxi.store();
         ppc.store();
-        if (t < P2) {
-            x0[qp] = v0;
-            x1[qp] = v1;
-        }
-        for (int r = t + bs; r < P2; r += bs) {
-            x0[r] = __hip_atomic_load(r0 + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            x1[r] = __hip_atomic_load(r1 + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
         if (t < P2) {
             double *h = L.hc + (size_t)qq * 6;
             h[0] = h0;
@@ -1375,6 +1359,19 @@ __device__ __forceinline__ void phi_chain(const Dev &d, int f, int i, uint32_t i
     /* chain i's cached logarithms after a start or restore */
     if (flag[0] == 0)
         for (int r = t; r < P; r += bs) phi_log_consts(L, r, xi[r], xi[P + r], pc + (size_t)r * 2);
+    /* the partners' rows. Every chain group in one launch: the earlier groups'
+     * chains after their step of this iteration (device-scope loads; the
+     * statistics above did not need them), and the later groups' chains wait
+     * for this one before they change theirs. */
+    if (chained && g > 0) wait_chains(d, f, 0, group_lo(g, n, G), iter, false);
+    {
+        const double *r0 = d.phi + (off + m0) * P2, *r1 = d.phi + (off + m1) * P2;
+        for (int r = t; r < P2; r += bs) {
+            x0[r] = chained ? __hip_atomic_load(r0 + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : r0[r];
+            x1[r] = chained ? __hip_atomic_load(r1 + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : r1[r];
+        }
+    }
+    __syncthreads();
     /* the proposal: jitter of parameter e is word e & 3 of Philox block 1 + e / 4 */
     for (int e = t; e < P2; e += bs) {
         const U4 r = st.block((uint32_t)i, 1u + (uint32_t)(e >> 2));
@@ -1541,6 +1538,26 @@ __device__ __forceinline__ void wait_run(const Dev &d, const SweepArgs &a, int f
     __syncthreads();
 }
 
+/* Several iterations per launch: what chain i's population step of the next
+ * iteration reads or overwrites is done with iteration iter: every chain's
+ * population step (the partners' rows, and chain i's row that they read) and,
+ * for a hierarchical run, in every subject the item holding chain i (chain
+ * i's subject values, and the priors it read from chain i's row; a migrating
+ * subject's ring marks every range). One flag per thread. */
+__device__ __forceinline__ void wait_chain_run(const Dev &d, const SweepArgs &a, int f, int i, uint32_t iter) {
+    const int n = d.nchain, G = d.G;
+    const JobDev job = d.phi_jobs[f];
+    int g = 0;
+    while (group_lo(g + 1, n, G) <= i) ++g;
+    const int per = G * a.rmax;
+    const uint32_t *ready = d.item_ready + (size_t)job.first_subject * per + (size_t)g * a.rmax +
+                            (i - group_lo(g, n, G)) / a.kmax;
+    const int ns = job.kind == GGDMC_KIND_HIER ? job.nsubject : 0;
+    for (int e = threadIdx.x; e < n + ns; e += blockDim.x)
+        wait_flag(d, e < n ? d.phi_ready + (size_t)f * n + e : ready + (size_t)(e - n) * per, iter);
+    __syncthreads();
+}
+
 /* Prior terms of proposals L.prop[0..k) for chains i0 + c under the
  * population state just published (read with device-scope loads): the
  * operations of proposal_stage1/2 (location, scale and cached logarithms per
@@ -1597,7 +1614,7 @@ __global__ __launch_bounds__(kBlock, 2) void subject_sweep_kernel(Dev d, SweepAr
         const int f = wid / nc, i = c_lo + wid % nc;
         const uint32_t it = *d.iter_base + a.iter_off + (uint32_t)seg;
         if (!phi_full(d, d.phi_jobs[f], it)) {
-            if (MULTI && seg > 0) wait_run(d, a, f, it - 1);
+            if (MULTI && seg > 0) wait_chain_run(d, a, f, i, it - 1);
             phi_chain<MULTI>(d, f, i, it, lds_raw, a.chained != 0);
         } else if (gb == 0 && i == 0) {
             if (MULTI && seg > 0) {
