@@ -516,7 +516,8 @@ __device__ __forceinline__ void proposals_loglike(const Dev &d, const SweepLds &
             if (to_global) {
                 for (int e = threadIdx.x; e < kb * nbat; e += blockDim.x) {
                     const int c = e / nbat, jj = e - c * nbat;
-                    d.csum[(size_t)(cbase + j_lo + jb + jj) * n + i0 + cb + c] = L.csum[(size_t)c * cmax + jj];
+                    __hip_atomic_store(d.csum + (size_t)(cbase + j_lo + jb + jj) * n + i0 + cb + c,
+                                       L.csum[(size_t)c * cmax + jj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
             } else {
                 for (int c = threadIdx.x; c < kb; c += blockDim.x) {
@@ -1831,8 +1832,12 @@ __global__ __launch_bounds__(kBlock, 2) void subject_sweep_kernel(Dev d, SweepAr
                 atomicAdd(d.evals + x, (unsigned long long)na * (unsigned long long)(t_hi - t_lo));
             }
             if (__builtin_expect(ntile > 1, 0)) {
-                /* the last-arriving tile of (subject, chain range) accepts */
-                __threadfence();
+                /* the last-arriving tile of (subject, chain range) accepts: the
+                 * tiles' chunk sums are device-scope stores, complete before
+                 * the counter moves, and read with device-scope loads (no
+                 * fence: a fence writes back and invalidates the XCD's L2,
+                 * costly when every tile does it) */
+                __builtin_amdgcn_s_waitcnt(0);
                 __syncthreads();
                 uint32_t *ctr = d.tile_ctr + ((size_t)x * G + g) * a.rmax + rng;
                 if (threadIdx.x == 0) {
@@ -1841,12 +1846,11 @@ __global__ __launch_bounds__(kBlock, 2) void subject_sweep_kernel(Dev d, SweepAr
                 }
                 __syncthreads();
                 if (!L.flag[3]) continue;
-                __threadfence();
                 if (threadIdx.x == 0) __hip_atomic_store(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 const int64_t cb = d.chunk_off[x], ncb = d.chunk_off[x + 1] - cb;
                 for (int c = threadIdx.x; c < k; c += blockDim.x) {
                     double s = 0.0;
-                    for (int64_t j = 0; j < ncb; ++j) s += d.csum[(size_t)(cb + j) * n + i0 + c];
+                    for (int64_t j = 0; j < ncb; ++j) s += ld<true>(d.csum + (size_t)(cb + j) * n + i0 + c);
                     L.tll[c] = s;
                 }
                 __syncthreads();
