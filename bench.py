@@ -50,11 +50,11 @@ W_ALG = {2: 130.0, 4: 255.0}
 # a launch (profiles/), at the decomposition the autotuner picks there (c3: 2
 # chains x all trials per item, c5: a whole chain group per item; every chain
 # group and 32 iterations in one launch).
-W_ISSUED = {"c3": 766.0, "c5": 840.0}
+W_ISSUED = {"c3": 751.0, "c5": 838.5}
 # HBM bytes per iteration of the subject sweep kernel at that decomposition:
 # rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors of a 256 MiB copy
 # kernel; profiles/) per launch, divided by the 32 iterations of a launch.
-TRAFFIC = {"c3": 0.987e6, "c5": 2.265e7}
+TRAFFIC = {"c3": 1.131e6, "c5": 2.124e7}
 # 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
 VALU_PEAK_TSLOTS = 256 * 4 * 16 * 2.4e9 / 1e12
 
