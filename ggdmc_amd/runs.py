@@ -149,6 +149,10 @@ def make_config(nmc, nchain, thin=1, pop_migration_prob=0.0, sub_migration_prob=
                 gamma_precursor=2.38, rp=0.001, is_hblocked=False, is_pblocked=False,
                 precision=_lib.MIXED_BATCHED, sweep_groups=0, trials_per_tile=0,
                 chains_per_item=0, launch_mode=0, iter_offset=0):
+    """ggdmc_config. trials_per_tile / chains_per_item / launch_mode 0 let the
+    library autotune the work split; launch_mode 1 = a subject launch per chain
+    group, 2 = one for every group, 3 = one for every group and 32 iterations.
+    None of them changes a result bit."""
     return _lib.Config(int(nmc), int(thin), int(nchain), float(pop_migration_prob),
                        float(sub_migration_prob), float(gamma_precursor), float(rp),
                        int(bool(is_hblocked)), int(bool(is_pblocked)), int(precision),
