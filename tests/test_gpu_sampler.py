@@ -279,3 +279,34 @@ def test_hier_with_an_empty_subject_follows_oracle(gpu):
         _same_hier(r, o)
         assert np.all(r["subject_theta"][1].log_likelihoods == 0.0)
         assert not np.array_equal(r["subject_theta"][1].theta[:, :, -1], subs[1][0])
+
+
+@pytest.mark.parametrize("launch_mode", [2, 3])
+def test_advance_in_pieces_equals_one_advance(gpu, launch_mode):
+    # a session advanced in uneven pieces (a restart's pattern; with several
+    # iterations per launch the pieces end mid-launch) gives the bits of one
+    # advance over the same iterations, stored columns included
+    fix = fx.load("lba_data6")
+    S, n = 4, 10
+    (phi, plp, pll), subs = fx.hier_start(fix, S, n, seed=23)
+    datas = [fix.pop_data(s) for s in range(S)]
+    cfg = gpu.make_config(nmc=29, thin=2, nchain=n, pop_migration_prob=0.1, sub_migration_prob=0.1,
+                          precision=gpu.MIXED_BATCHED, sweep_groups=2, chains_per_item=2,
+                          launch_mode=launch_mode)
+    job = gpu.JobSpec(gpu.KIND_HIER, 77, fix.pop_prior, h_prior=fix.hyper_prior, subjects=datas,
+                      phi_start=(phi, plp, pll), subject_starts=subs, nparameter_de=2 * fix.P)
+    out = []
+    for pieces in ((57,), (5, 7, 13, 31, 1)):
+        s = gpu.Session(cfg, fix.model, [job])
+        for p in pieces:
+            s.advance(p)
+        out.append(s.download()[0])
+        s.close()
+    a, b = out
+    assert np.array_equal(a["phi"].theta, b["phi"].theta)
+    assert np.array_equal(a["phi"].log_likelihoods, b["phi"].log_likelihoods)
+    for x, y in zip(a["subjects"], b["subjects"]):
+        assert np.array_equal(x.theta, y.theta)
+        assert np.array_equal(x.log_likelihoods, y.log_likelihoods)
+        assert np.array_equal(x.summed_log_prior, y.summed_log_prior)
+    assert not np.array_equal(a["subjects"][0].theta[:, :, -1], subs[0][0])
