@@ -310,3 +310,73 @@ def test_advance_in_pieces_equals_one_advance(gpu, launch_mode):
         assert np.array_equal(x.log_likelihoods, y.log_likelihoods)
         assert np.array_equal(x.summed_log_prior, y.summed_log_prior)
     assert not np.array_equal(a["subjects"][0].theta[:, :, -1], subs[0][0])
+
+
+def test_hblocked_population_in_every_launch_mode(gpu):
+    # blocked population sweeps: every iteration takes the whole-run population
+    # step (one workgroup of the launch); with several iterations per launch it
+    # waits for the whole run's previous iteration. Follows the oracle, and
+    # every launch mode gives the same bits.
+    fix = fx.load("lba_data6")
+    S, n = 4, 10
+    (phi, plp, pll), subs = fx.hier_start(fix, S, n, seed=29)
+    datas = [fix.pop_data(s) for s in range(S)]
+    ref = None
+    for lm in (1, 2, 3):
+        cfg = gpu.make_config(nmc=6, nchain=n, pop_migration_prob=0.2, sub_migration_prob=0.2,
+                              precision=gpu.FP64_BATCHED, sweep_groups=2, is_hblocked=True,
+                              chains_per_item=2, launch_mode=lm)
+        r = gpu.run(cfg, fix.model, fix.pop_prior, fix.hyper_prior, datas, (phi, plp, pll), subs,
+                    seed=91)
+        if ref is None:
+            ocfg = _ocfg(fix, cfg, nparameter_de=2 * fix.P, seed=91)
+            ocfg.sweep_groups = 2
+            rt = np.concatenate([d.rt for d in datas])
+            so = np.concatenate([[0], np.cumsum([len(d.rt) for d in datas])])
+            co = np.stack([d.cell_off for d in datas])
+            o = O.run_hier(ocfg, fix.omodel, fx.oprior(fix.pop_prior), fx.oprior(fix.hyper_prior),
+                           rt, so, co, phi, plp, pll, np.stack([s[0] for s in subs]),
+                           np.stack([s[1] for s in subs]), np.stack([s[2] for s in subs]))
+            _same_hier(r, o)
+            ref = r
+            continue
+        assert np.array_equal(r["phi"].theta, ref["phi"].theta), lm
+        assert np.array_equal(r["phi"].log_likelihoods, ref["phi"].log_likelihoods), lm
+        for a, b in zip(r["subject_theta"], ref["subject_theta"]):
+            assert np.array_equal(a.theta, b.theta), lm
+            assert np.array_equal(a.log_likelihoods, b.log_likelihoods), lm
+
+
+def test_hyper_and_hierarchical_batch_in_every_launch_mode(gpu):
+    # a hyper-only run (fixed data rows, whole-run population step every
+    # iteration) in one batch with a hierarchical run: every launch mode gives
+    # the same bits
+    fix = fx.load("lba_data6")
+    S, n, P = 4, 10, fix.P
+    (phi, plp, pll), subs = fx.hier_start(fix, S, n, seed=31)
+    datas = [fix.pop_data(s) for s in range(S)]
+    rng = np.random.default_rng(5)
+    hdata = fx.truth_vector(fix)[None, :] * np.exp(rng.normal(0, 0.1, (24, P)))
+    hp, pp = fx.oprior(fix.hyper_prior), fx.oprior(fix.pop_prior)
+    hlp = np.array([O.sumlogprior(hp, phi[:, c]) for c in range(n)])
+    hll = np.array([O.sumloghlike(pp, phi[:, c], hdata) for c in range(n)])
+    jobs = [gpu.JobSpec(gpu.KIND_HIER, 41, fix.pop_prior, h_prior=fix.hyper_prior, subjects=datas,
+                        phi_start=(phi, plp, pll), subject_starts=subs, nparameter_de=2 * P),
+            gpu.JobSpec(gpu.KIND_HYPER, 43, fix.pop_prior, h_prior=fix.hyper_prior,
+                        phi_start=(phi, hlp, hll), hyper_data=hdata, nparameter_de=2 * P)]
+    ref = None
+    for lm in (1, 2, 3):
+        cfg = gpu.make_config(nmc=6, nchain=n, pop_migration_prob=0.2, sub_migration_prob=0.2,
+                              precision=gpu.MIXED_BATCHED, sweep_groups=2, chains_per_item=2,
+                              launch_mode=lm)
+        res = gpu.run_batch(cfg, fix.model, jobs)
+        if ref is None:
+            ref = res
+            continue
+        for a, b in zip(res, ref):
+            assert np.array_equal(a["phi"].theta, b["phi"].theta), lm
+            assert np.array_equal(a["phi"].log_likelihoods, b["phi"].log_likelihoods), lm
+            for x, y in zip(a["subjects"], b["subjects"]):
+                assert np.array_equal(x.theta, y.theta), lm
+                assert np.array_equal(x.log_likelihoods, y.log_likelihoods), lm
+    assert not np.array_equal(ref[1]["phi"].theta[:, :, -1], phi)
