@@ -4,8 +4,9 @@
  * A session uploads a batch of runs once (trials, designs, priors, start
  * states), keeps every state in HBM, and advances all runs together:
  *   batched path : per iteration one subject launch per (sweep, chain group)
- *                  over work items (subject, chain range, trial tile), or one
- *                  per sweep with one item per subject over every group; the
+ *                  over work items (subject, chain range, trial tile), one
+ *                  for every group, one for every group and 32 iterations,
+ *                  or one per sweep with one item per subject over every group; the
  *                  population step runs inside the subject launches (one
  *                  workgroup per chain of the launch's group) or, with one
  *                  item per subject or blocked subject sweeps, in its own

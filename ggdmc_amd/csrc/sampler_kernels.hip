@@ -8,7 +8,12 @@
  *                        item itself or by the last-arriving tile; migration ring
  *                        (src/de.cpp:615-665 / :157-199) and the deferred
  *                        thinned store of the previous iteration, every item
- *                        storing the chains it updates.
+ *                        storing the chains it updates. Leading workgroups run
+ *                        the population step of the launch's chains; with
+ *                        every chain group in one launch later groups wait for
+ *                        earlier ones, and with several iterations per launch
+ *                        (MULTI) each iteration's segment waits for the
+ *                        previous one, subject by subject.
  *  phi_kernel            per-(chain, parameter) sufficient statistics of the
  *                        subject states, then the population-level step
  *                        (src/de.cpp:385-465, :467-565) with sumloghlike
