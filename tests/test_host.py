@@ -244,3 +244,22 @@ def test_psrf_and_mcse_on_known_processes():
     se = diagnostics.mcse(x[None, :, :])[0]
     want = 1 / (1 - rho) / np.sqrt(n * m)
     assert abs(se / want - 1) < 0.15
+
+
+# -------------------------------------------------------------- workloads
+def test_replicated_workload_shares_data_with_new_keys():
+    # independent runs of one workload (bench.py --runs): same data and start
+    # state, Philox keys seed + r, the run order kept job by job
+    from ggdmc_amd import workloads
+
+    def ll(model, data, thetas):
+        om = O.Model(model.winner, model.par_src, model.par_const, model.npar)
+        return np.array([O.lba_loglike(om, data.rt, data.cell_off, t) for t in thetas])
+
+    w = workloads.make("c3", scale=1 / 16, ll_fn=ll)
+    r = workloads.replicate(w, 3)
+    assert len(r.jobs) == 3 * len(w.jobs)
+    assert [j.seed for j in r.jobs] == [w.jobs[0].seed + k for k in range(3)]
+    assert all(j.subjects is w.jobs[0].subjects and j.phi_start is w.jobs[0].phi_start for j in r.jobs)
+    assert r.n_subjects == 3 * w.n_subjects and r.n_trials == 3 * w.n_trials
+    assert workloads.replicate(w, 1) is w
