@@ -172,7 +172,7 @@ def main():
         calibrate_copy()
         return
     if a.gpus != 1:
-        print("bench.py measures one GPU: this build runs no multi-GPU configuration "
+        print("bench.py measures one GPU; the engine runs no multi-GPU configuration "
               "(--gpus %d requested)" % a.gpus, file=sys.stderr)
         sys.exit(2)
     import ggdmc_amd as g
