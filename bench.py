@@ -50,11 +50,17 @@ W_ALG = {2: 130.0, 4: 255.0}
 # a launch (profiles/), at the decomposition the autotuner picks there (c3: 2
 # chains x all trials per item, c5: a whole chain group per item; every chain
 # group and 32 iterations in one launch).
-W_ISSUED = {"c3": 751.0, "c5": 838.5}
+# Keyed by (workload, chains per item, trials per item, launch mode): a figure
+# is reported only for the decomposition it was counted on.
+W_ISSUED = {("c3", 2, 0, 3): 751.0, ("c5", 17, 0, 3): 838.5}
 # HBM bytes per iteration of the subject sweep kernel at that decomposition:
 # rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors of a 256 MiB copy
 # kernel; profiles/) per launch, divided by the 32 iterations of a launch.
-TRAFFIC = {"c3": 1.131e6, "c5": 2.124e7}
+TRAFFIC = {("c3", 2, 0, 3): 1.131e6, ("c5", 17, 0, 3): 2.124e7}
+# Iterations a bench session holds at least, whatever --steps asks: the
+# library autotunes the work decomposition of sessions of 64 or more
+# iterations, so the measured configuration never depends on the step count.
+MIN_SESSION_ITERS = 128
 # 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
 VALU_PEAK_TSLOTS = 256 * 4 * 16 * 2.4e9 / 1e12
 
@@ -186,8 +192,9 @@ def main():
     w = workloads.replicate(w_one, a.runs)
     n_acc = w.model.n_acc
     total = a.warmup + a.steps + a.profile_iters
-    thin = max(1, total // 1000)
-    nmc = total // thin + 2
+    sess_iters = max(total, MIN_SESSION_ITERS)
+    thin = max(1, sess_iters // 1000)
+    nmc = sess_iters // thin + 2
     if a.decomp:
         dv = [int(v) for v in a.decomp.split(",")]
         k, t, m = dv[0], dv[1], (dv[2] if len(dv) > 2 else 1)
@@ -201,7 +208,8 @@ def main():
           "%d per launch, %s; setup %.1fs"
           % (w.name, w.description, w.n_subjects, w.n_trials,
              "all groups" if kci < 0 else kci, tpt or "all", nitems,
-             "one subject launch for every chain group" if lmode == 2 else "a subject launch per chain group",
+             {1: "a subject launch per chain group", 2: "one subject launch for every chain group",
+              3: "one subject launch for every chain group and 32 iterations"}[lmode],
              time.time() - t_build),
           file=sys.stderr, flush=True)
 
@@ -223,13 +231,14 @@ def main():
     evals_per_launch = (e3 - e2) / max(n_launch, 1)
     iters_per_launch = a.profile_iters / max(n_launch, 1)
     achieved = evals_per_launch * W_ALG[n_acc] / (k_ms / 1e3) / 1e12
+    key = (w.name, kci, tpt, lmode)
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TSLOTS, 2),
                 "unit": "Tslot/s", "frac": round(achieved / VALU_PEAK_TSLOTS, 4),
-                "traffic": TRAFFIC[w.name] * iters_per_launch if w.name in TRAFFIC else None,
+                "traffic": TRAFFIC[key] * iters_per_launch if key in TRAFFIC else None,
                 "kernel": "subject_sweep_kernel", "iterations_per_launch": iters_per_launch,
                 "kernel_ms": round(k_ms, 4), "evals_per_launch": evals_per_launch,
                 "slots_per_eval": W_ALG[n_acc]}
-    wi = W_ISSUED.get(w.name)
+    wi = W_ISSUED.get(key)
     if wi:
         issued = evals_per_launch * wi / (k_ms / 1e3) / 1e12
         roofline.update({"issued_slots_per_eval": wi, "issued": round(issued, 3),

@@ -775,10 +775,6 @@ void ggdmc_session::run_batched(uint32_t n_iter) {
 
 void ggdmc_session::run_plain(uint32_t n_iter) {
     int nsw_phi = 0;
-    for (int f = 0; f < d.n_phi; ++f) {
-        const ggdmc_job &j = jobs[phi_of_job.empty() ? 0 : 0];
-        (void)j;
-    }
     if (d.n_phi > 0) {
         const bool any_blocked = cfg.is_hblocked || cfg.is_pblocked;
         nsw_phi = any_blocked ? 2 * d.P : 1;

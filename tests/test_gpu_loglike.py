@@ -43,11 +43,9 @@ def test_mixed_matches_goldens(gpu, name):
     e = np.concatenate(errs)
     print("%s: %d golden vectors, mixed max rel %.2e, %d above %.0e"
           % (name, len(e), e.max(), int((e > MIXED_RTOL).sum()), MIXED_RTOL))
-    # prior-draw vectors include survivors at the FP64 rounding floor, where
-    # the reference's 1 - F is itself quantised; all but a handful agree to
-    # the tolerance and none is off by more than 1e-4
-    assert (e <= MIXED_RTOL).mean() >= 0.998
-    assert e.max() <= 1e-4
+    # every golden vector, prior draws with survivors at the FP64 rounding
+    # floor included, within the per-proposal tolerance
+    assert e.max() <= MIXED_RTOL
 
 
 def test_mixed_near_posterior(gpu):
