@@ -247,3 +247,262 @@ def test_normal_mean_posterior(groups):
         # 500 effective draws is conservative for 12 chains x 3500 iterations
         assert abs(draws[c].mean() - mean) < 4 * sd / np.sqrt(500)
         assert abs(draws[c].std() / sd - 1) < 0.1
+
+
+# ------------------------------------------- hierarchical / hyper-only samplers
+# The oracle's run_hier / run_hyper (oracle/oracle.cpp) against a brute-force
+# pure-Python replay written from src/de.cpp (tests/replay.py), which
+# re-derives every proposal and accept from the Philox words and the previous
+# state; and against invariants of the reference's wiring that a plausible
+# mistake (no refresh, a refreshed subject prior, the destination's context in
+# migration, subjects before the population) breaks.
+import replay as R  # noqa: E402
+
+
+def test_replay_philox_known_answers():
+    assert R.philox((0, 0, 0, 0), (0, 0)) == (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)
+    assert R.philox((0xffffffff,) * 4, (0xffffffff,) * 2) == \
+        (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)
+
+
+def _tiny_hier(nchain=5, S=2, per_cell=2, seed=21):
+    """lba_data6 design, S subjects of per_cell trials per cell, a start near
+    the generating values (phi [2P, n], subjects [S, P, n] with lp under phi
+    chain c and ll from the oracle)."""
+    fix = fx.load("lba_data6")
+    P = fix.P
+    rts, offs = [], []
+    for s in range(S):
+        d = fix.pop_data(s)
+        co = d.cell_off
+        take = [np.asarray(d.rt[co[c]:co[c + 1]][:per_cell]) for c in range(len(co) - 1)]
+        rts.append(np.concatenate(take))
+        offs.append(np.concatenate([[0], np.cumsum([len(t) for t in take])]).astype(np.int64))
+    rng = np.random.default_rng(seed)
+    tv = fx.truth_vector(fix)
+    sc = np.array([fx.BV_SCALE[p] for p in fix.model.pnames])
+    phi = np.concatenate([tv[:, None] * np.exp(rng.normal(0, 0.1, (P, nchain))),
+                          sc[:, None] * np.exp(rng.normal(0, 0.1, (P, nchain)))])
+    pp, hp = fix.pop_prior, fix.hyper_prior
+    ppr = (pp.dist, pp.p0, pp.p1, pp.lower, pp.upper)
+    hpr = (hp.dist, hp.p0, hp.p1, hp.lower, hp.upper)
+    sub = np.empty((S, P, nchain))
+    slp = np.empty((S, nchain))
+    sll = np.empty((S, nchain))
+    for s in range(S):
+        sub[s] = tv[:, None] * np.exp(rng.normal(0, 0.1, (P, nchain)))
+        for c in range(nchain):
+            slp[s, c] = R.sumlogprior(R.phi_prior(phi[:, c], ppr), sub[s, :, c])
+            sll[s, c] = O.lba_loglike(fix.omodel, rts[s], offs[s], sub[s, :, c])
+    plp = np.array([R.sumlogprior(hpr, phi[:, c]) for c in range(nchain)])
+    pll = np.array([R.sumloghlike(ppr, phi[:, c], [sub[s, :, c] for s in range(S)])
+                    for c in range(nchain)])
+    return dict(fix=fix, P=P, S=S, n=nchain, rts=rts, offs=offs, phi=phi, plp=plp, pll=pll,
+                sub=sub, slp=slp, sll=sll, ppr=ppr, hpr=hpr)
+
+
+def _oracle_hier(t, cfg):
+    rt = np.concatenate(t["rts"])
+    so = np.concatenate([[0], np.cumsum([len(r) for r in t["rts"]])]).astype(np.int64)
+    co = np.stack(t["offs"])
+    fix = t["fix"]
+    return O.run_hier(cfg, fix.omodel, fx.oprior(fix.pop_prior), fx.oprior(fix.hyper_prior), rt,
+                      so, co, t["phi"], t["plp"], t["pll"], t["sub"], t["slp"], t["sll"])
+
+
+def _replay_hier(t, cfg, nmc):
+    fix = t["fix"]
+    rcfg = dict(seed=cfg.seed, iter_offset=cfg.iter_offset, rp=cfg.rp,
+                gamma=cfg.gamma_precursor / np.sqrt(2.0 * cfg.nparameter_de),
+                pop_mig=cfg.pop_migration_prob, sub_mig=cfg.sub_migration_prob,
+                groups=cfg.sweep_groups, hblocked=bool(cfg.is_hblocked),
+                pblocked=bool(cfg.is_pblocked))
+    phi = R.Level(t["phi"], t["plp"], t["pll"])
+    subs = [R.Level(t["sub"][s], t["slp"][s], t["sll"][s]) for s in range(t["S"])]
+
+    def sll(s, x):
+        return O.lba_loglike(fix.omodel, t["rts"][s], t["offs"][s], x)
+
+    states = [(phi.copy(), [x.copy() for x in subs])]
+    for it in range(1, nmc):
+        R.hier_iteration(rcfg, phi, subs, t["ppr"], t["hpr"], sll, it)
+        states.append((phi.copy(), [x.copy() for x in subs]))
+    return states
+
+
+def _same_as_replay(out, states):
+    pth, plp, pll, sth, slp, sll = out
+    for k, (phi, subs) in enumerate(states):
+        np.testing.assert_allclose(pth[:, :, k], phi.th, rtol=1e-13, atol=0, err_msg="phi %d" % k)
+        np.testing.assert_allclose(plp[:, k], phi.lp, rtol=1e-13, err_msg="phi lp %d" % k)
+        np.testing.assert_allclose(pll[:, k], phi.ll, rtol=1e-12, err_msg="phi ll %d" % k)
+        for s, x in enumerate(subs):
+            np.testing.assert_allclose(sth[s, :, :, k], x.th, rtol=1e-13, atol=0,
+                                       err_msg="subject %d, %d" % (s, k))
+            np.testing.assert_allclose(slp[s, :, k], x.lp, rtol=1e-12, err_msg="lp %d, %d" % (s, k))
+            np.testing.assert_allclose(sll[s, :, k], x.ll, rtol=1e-13, err_msg="ll %d, %d" % (s, k))
+
+
+def _moves(out):
+    """Fractions of (chain, iteration) steps in which the population and the
+    subject states changed."""
+    pth, _, _, sth, _, _ = out
+    pm = np.mean(np.any(np.diff(pth, axis=2) != 0, axis=0))
+    sm = np.mean(np.any(np.diff(sth, axis=3) != 0, axis=1))
+    return pm, sm
+
+
+@pytest.mark.parametrize("groups,mig", [(5, 0.25), (2, 0.25), (5, 1.0)])
+def test_oracle_hier_follows_the_replay(groups, mig):
+    t = _tiny_hier()
+    nmc = 30
+    cfg = O.config(nmc=nmc, nchain=t["n"], nparameter_de=2 * t["P"], sweep_groups=groups,
+                   pop_migration_prob=mig, sub_migration_prob=mig, seed=77, iter_offset=3)
+    out = _oracle_hier(t, cfg)
+    _same_as_replay(out, _replay_hier(t, cfg, nmc))
+    pm, sm = _moves(out)
+    assert pm > 0.05 and sm > 0.05, (pm, sm)     # the trajectories exercise accepts
+
+
+def test_oracle_hier_blocked_follows_the_replay():
+    t = _tiny_hier(S=2, per_cell=1)
+    nmc = 6
+    cfg = O.config(nmc=nmc, nchain=t["n"], nparameter_de=2 * t["P"], sweep_groups=2,
+                   pop_migration_prob=0.1, sub_migration_prob=0.1, is_hblocked=True,
+                   is_pblocked=True, seed=5)
+    out = _oracle_hier(t, cfg)
+    _same_as_replay(out, _replay_hier(t, cfg, nmc))
+    pm, sm = _moves(out)
+    assert pm > 0.05 and sm > 0.05, (pm, sm)
+
+
+@pytest.mark.parametrize("blocked,mig", [(False, 0.3), (True, 0.3)])
+def test_oracle_hyper_follows_the_replay(blocked, mig):
+    t = _tiny_hier(nchain=6, S=8)
+    data = np.stack([t["sub"][s][:, 0] for s in range(t["S"])])     # [S, P] subject vectors
+    fix = t["fix"]
+    nmc = 25 if not blocked else 6
+    cfg = O.config(nmc=nmc, nchain=t["n"], nparameter_de=2 * t["P"], sweep_groups=t["n"],
+                   sub_migration_prob=mig, is_pblocked=blocked, seed=13)
+    pll = np.array([R.sumloghlike(t["ppr"], t["phi"][:, c], list(data)) for c in range(t["n"])])
+    th, lp, ll = O.run_hyper(cfg, fx.oprior(fix.pop_prior), fx.oprior(fix.hyper_prior), data,
+                             t["phi"], t["plp"], pll)
+    rcfg = dict(seed=cfg.seed, iter_offset=0, rp=cfg.rp, mig=mig, groups=t["n"],
+                gamma=cfg.gamma_precursor / np.sqrt(2.0 * cfg.nparameter_de), blocked=blocked)
+    states = R.run_single(rcfg, R.Level(t["phi"], t["plp"], pll),
+                          lambda x, i: R.sumlogprior(t["hpr"], x),
+                          lambda x, i: R.sumloghlike(t["ppr"], x, list(data)), nmc)
+    for k, L in enumerate(states):
+        np.testing.assert_allclose(th[:, :, k], L.th, rtol=1e-13, atol=0)
+        np.testing.assert_allclose(lp[:, k], L.lp, rtol=1e-13)
+        np.testing.assert_allclose(ll[:, k], L.ll, rtol=1e-12)
+    assert np.mean(np.any(np.diff(th, axis=2) != 0, axis=0)) > 0.05
+
+
+def test_oracle_subject_run_follows_the_replay():
+    fix = fx.load("lba_data6")
+    start = fx.near_start(fix, 7, seed=3)
+    nmc = 25
+    cfg = O.config(nmc=nmc, nchain=7, nparameter_de=fix.P, sweep_groups=7,
+                   sub_migration_prob=0.3, seed=99)
+    th, lp, ll = O.run_subject(cfg, fix.omodel, fix.oprior, fix.sub_data.rt,
+                               fix.sub_data.cell_off, *start)
+    sp = fix.sub_prior
+    rcfg = dict(seed=99, iter_offset=0, rp=cfg.rp, mig=0.3, groups=7, blocked=False,
+                gamma=2.38 / np.sqrt(2.0 * fix.P))
+    states = R.run_single(rcfg, R.Level(*start),
+                          lambda x, i: R.sumlogprior((sp.dist, sp.p0, sp.p1, sp.lower, sp.upper), x),
+                          lambda x, i: O.lba_loglike(fix.omodel, fix.sub_data.rt,
+                                                     fix.sub_data.cell_off, x), nmc)
+    for k, L in enumerate(states):
+        np.testing.assert_allclose(th[:, :, k], L.th, rtol=1e-13, atol=0)
+        np.testing.assert_allclose(ll[:, k], L.ll, rtol=1e-13)
+    assert np.mean(np.any(np.diff(th, axis=2) != 0, axis=0)) > 0.05
+
+
+def test_oracle_hier_reference_wiring_invariants():
+    """Wiring of run_hchains, column by column of one oracle run (crossover
+    only; thin 1): column k holds the state after iteration k."""
+    t = _tiny_hier(nchain=6, S=3)
+    nmc = 40
+    cfg = O.config(nmc=nmc, nchain=t["n"], nparameter_de=2 * t["P"], sweep_groups=t["n"],
+                   seed=8)
+    pth, plp, pll, sth, slp, sll = _oracle_hier(t, cfg)
+    S, n = t["S"], t["n"]
+    stale = 0
+    for k in range(1, nmc):
+        for c in range(n):
+            # the population step refreshes chain c's likelihood from the
+            # subjects' columns of the previous iteration before it uses it
+            # (src/de.cpp:397-398): whatever it decided, the stored ll is the
+            # hyper likelihood of its state against those columns
+            prev_cols = [sth[s, :, c, k - 1] for s in range(S)]
+            assert abs(pll[c, k] - R.sumloghlike(t["ppr"], pth[:, c, k], prev_cols)) <= 1e-11
+            for s in range(S):
+                x = sth[s, :, c, k]
+                under_new = R.sumlogprior(R.phi_prior(pth[:, c, k], t["ppr"]), x)
+                if np.any(x != sth[s, :, c, k - 1]):
+                    # an accepted subject proposal takes its prior under this
+                    # iteration's population state: the population step ran
+                    # first (src/de.cpp:283-374, :599-600)
+                    assert abs(slp[s, c, k] - under_new) <= 1e-11
+                else:
+                    # a rejected one keeps the cached lp, not refreshed when
+                    # the population state moved (src/de.cpp:577, :606)
+                    assert slp[s, c, k] == slp[s, c, k - 1]
+                    if np.any(pth[:, c, k] != pth[:, c, k - 1]) and \
+                            abs(slp[s, c, k] - under_new) > 1e-9 * abs(under_new):
+                        stale += 1
+    assert stale > 10
+
+
+def test_oracle_hier_migration_takes_the_source_context():
+    """Migration every iteration at both levels (src/de.cpp:467-565,
+    :615-665). Population ring over the sorted subset S: step k refreshes the
+    cached likelihoods of S_k and S_k+1 from their own subject columns, then
+    proposes a jittered copy of S_k for S_k+1 whose likelihood is taken against
+    S_k's subject columns. So after the ring: chains outside S keep their
+    cached likelihood; S_1.. (each refreshed again as the source of the next
+    step) hold the hyper likelihood of their state against their own columns;
+    S_0, written by the last step, holds, if that step accepted, the hyper
+    likelihood against the last member's columns. Subject ring: an accepted
+    copy's prior is under the source chain's population state."""
+    t = _tiny_hier(nchain=6, S=2)
+    nmc = 30
+    cfg = O.config(nmc=nmc, nchain=t["n"], nparameter_de=2 * t["P"], sweep_groups=t["n"],
+                   pop_migration_prob=1.0, sub_migration_prob=1.0, rp=0.05, seed=4)
+    pth, plp, pll, sth, slp, sll = _oracle_hier(t, cfg)
+    S, n = t["S"], t["n"]
+    seen_pop = seen_sub = 0
+
+    def hll(c, k, cols_chain):
+        return R.sumloghlike(t["ppr"], pth[:, c, k], [sth[s, :, cols_chain, k - 1] for s in range(S)])
+
+    for k in range(1, nmc):
+        ring = R.subset(R.Stream(cfg.seed, k, R.UNIT_PHI, 0), n)
+        for c in range(n):
+            if c not in ring:
+                assert np.array_equal(pth[:, c, k], pth[:, c, k - 1]) and pll[c, k] == pll[c, k - 1]
+        for j, c in enumerate(ring):
+            if j > 0:
+                assert abs(pll[c, k] - hll(c, k, c)) <= 1e-11
+            elif np.any(pth[:, c, k] != pth[:, c, k - 1]):
+                last = ring[-1]
+                want, own = hll(c, k, last), hll(c, k, c)
+                assert abs(pll[c, k] - want) <= 1e-11
+                if abs(want - own) > 1e-9:
+                    seen_pop += 1
+            else:
+                assert abs(pll[c, k] - hll(c, k, c)) <= 1e-11
+        for s in range(S):
+            ring = R.subset(R.Stream(cfg.seed, k, s, 0), n)
+            for j, dst in enumerate(ring):
+                src = ring[j - 1]
+                x = sth[s, :, dst, k]
+                if np.any(x != sth[s, :, dst, k - 1]):
+                    want = R.sumlogprior(R.phi_prior(pth[:, src, k], t["ppr"]), x)
+                    own = R.sumlogprior(R.phi_prior(pth[:, dst, k], t["ppr"]), x)
+                    assert abs(slp[s, dst, k] - want) <= 1e-11
+                    if abs(want - own) > 1e-9:
+                        seen_sub += 1
+    assert seen_pop > 3 and seen_sub > 3, (seen_pop, seen_sub)
