@@ -76,16 +76,20 @@ def initialise_theta(nchain, priors, dmi, seed=None, max_init_attempts=1000, bat
     return th, lp, ll
 
 
-def initialise_phi(nchain, priors, dmis, seed=None, max_init_attempts=1000):
+def initialise_phi(nchain, priors, dmis, seed=None, max_init_attempts=1000, rng=None):
     """R/phi.R:256-332: subjects from initialise_theta under p_prior (seed +
-    subject index), then phi chains from h_prior with finite log prior and
-    sumloghlike. Returns (phi start, [subject starts])."""
+    subject index, 1-based), then per chain the first h_prior draw with finite
+    log prior and sumloghlike against the subjects' start values of that chain;
+    an error when max_init_attempts draws of a chain all fail. The subjects'
+    start lp stays the p_prior density initialise_theta gave it (the reference
+    does not re-evaluate it under the phi chain it will be paired with).
+    Returns (phi start, [subject starts])."""
     if priors["h_prior"] is None:
         raise ValueError("hyper prior is NULL")
     seed = 0 if seed is None else int(seed)
     subs = [initialise_theta(nchain, priors, d, seed + s + 1, max_init_attempts)
             for s, d in enumerate(dmis)]
-    rng = np.random.default_rng(seed)
+    rng = rng if rng is not None else np.random.default_rng(seed)
     hp, pp = priors["h_prior"], priors["p_prior"]
     phi = np.full((hp.npar, nchain), np.nan)
     plp = np.full(nchain, -np.inf)
@@ -102,11 +106,6 @@ def initialise_phi(nchain, priors, dmis, seed=None, max_init_attempts=1000):
         else:
             raise RuntimeError("Chain %d: Failed to find valid phi after %d attempts"
                                % (c + 1, max_init_attempts))
-    # subject log priors under the phi chain they are paired with
-    for s in subs:
-        for c in range(nchain):
-            pc = PriorSpec.tnorm(phi[:pp.npar, c], phi[pp.npar:, c], pp.lower, pp.upper)
-            s[1][c] = sumlogprior(pc, s[0][:, c])
     return (phi, plp, pll), subs
 
 
@@ -202,40 +201,35 @@ def RestartSampling(samples_list, nmc=500, thin=1, **kw):
     return _restart(samples_list, nmc, thin, kw)
 
 
-def StartSampling_hyper(hyper_data, priors, samples_list=None, nmc=500, nchain=None, thin=1,
+def StartSampling_hyper(hyper_dmi, dmis, priors, samples_list=None, nmc=500, nchain=None, thin=1,
                         max_init_attempts=1000, sub_migration_prob=0.0, gamma_precursor=2.38,
                         rp=0.001, is_pblocked=False, ncore=3, seed=None,
                         precision=_lib.MIXED_BATCHED, sweep_groups=0):
     """R/sampling.R:615-676: population parameters of fixed subject vectors
-    hyper_data [S, P] (the hyper_dmi data)."""
+    hyper_dmi [S, P] (the hyper_dmi data). As in the reference, the start of
+    each run is initialise_phi's phi on the subjects `dmis`
+    (R/sampling.R:651-656), lp / ll as initialise_phi computed them, so an
+    exhausted initialisation raises."""
     nchain = _get_nchain(priors, nchain)
     seeds = _run_seeds(seed, ncore)
     hp, pp = priors["h_prior"], priors["p_prior"]
+    if hp is None:
+        raise ValueError("h_prior has not found")
     if samples_list is None:
-        samples_list = []
-        for sd in seeds:
-            rng = np.random.default_rng(sd)
-            phi = np.empty((hp.npar, nchain))
-            lp = np.empty(nchain)
-            ll = np.empty(nchain)
-            for c in range(nchain):
-                for _ in range(max_init_attempts):
-                    cand = rprior(hp, 1, rng)[0]
-                    a, b = sumlogprior(hp, cand), sumloghlike(pp, cand, hyper_data)
-                    if np.isfinite(a) and np.isfinite(b):
-                        phi[:, c], lp[c], ll[c] = cand, a, b
-                        break
-            samples_list.append((phi, lp, ll))
+        samples_list = [initialise_phi(nchain, priors, dmis, sd % 1000000, max_init_attempts)[0]
+                        for sd in seeds]
     kw = dict(sub_migration_prob=sub_migration_prob, gamma_precursor=gamma_precursor, rp=rp,
               is_pblocked=is_pblocked, precision=precision, sweep_groups=sweep_groups)
     cfg = _config(nmc, nchain, thin, kw)
-    jobs = [JobSpec(_lib.KIND_HYPER, sd, pp, h_prior=hp, phi_start=st,
-                    hyper_data=np.asarray(hyper_data, float), nparameter_de=priors["nparameter"])
-            for sd, st in zip(seeds, samples_list)]
+    hd = np.asarray(hyper_dmi, float)
+    # the start's cached likelihood is initialise_phi's (against the subjects'
+    # start values), handed to run_hyper unchanged as the reference does
+    jobs = [JobSpec(_lib.KIND_HYPER, sd, pp, h_prior=hp, phi_start=st, hyper_data=hd,
+                    nparameter_de=priors["nparameter"]) for sd, st in zip(seeds, samples_list)]
     res = _execute(_lib.KIND_HYPER, None, jobs, cfg)
     out = Fit(r["phi"] for r in res)
     out.configs = dict(kw, nmc=nmc, nchain=nchain, thin=thin, seeds=seeds)
-    out.dmis, out.priors, out.kind = hyper_data, priors, _lib.KIND_HYPER
+    out.dmis, out.priors, out.kind = hd, priors, _lib.KIND_HYPER
     out.iterations_done = nmc * thin - 1
     return out
 
@@ -284,10 +278,50 @@ def _restart(prev, nmc, thin, kw):
     return out
 
 
+def _param_array(posts):
+    """.GetParamArray (R/parallel_helper.R:15-53): run i's theta [npar, nchain,
+    nmc] flattened column-major to [npar, nchain * nmc] (chain fastest) becomes
+    column i of [npar, nruns, nchain * nmc]."""
+    p0 = posts[0]
+    npar, nchain, nmc = p0.theta.shape
+    out = np.empty((npar, len(posts), nchain * nmc))
+    for i, p in enumerate(posts):
+        out[:, i, :] = p.theta.reshape(npar, nchain * nmc, order="F")
+    return out
+
+
+def _like_matrix(posts, is_prior):
+    """.get_likelihood_matrix (R/parallel_helper.R:1-12): each run's
+    [nchain, nmc] matrix transposed and flattened column-major (iteration
+    fastest), one row per run: [nruns, nmc * nchain]. The order differs from
+    the parameter array's (chain fastest); kept as the reference has it."""
+    rows = [(p.summed_log_prior if is_prior else p.log_likelihoods).T.reshape(-1, order="F")
+            for p in posts]
+    return np.stack(rows)
+
+
+def _rebuild(posts):
+    p0 = posts[0]
+    return Posterior(_param_array(posts), _like_matrix(posts, True), _like_matrix(posts, False),
+                     p0.nmc * p0.nchain, p0.thin, len(posts), p0.pnames, p0.start)
+
+
 def RebuildPosterior(fits):
-    """R/parallel_helper.R:62-82: the ncore runs' chains side by side."""
-    ps = [f if isinstance(f, Posterior) else f["phi"] for f in fits]
-    th = np.concatenate([p.theta for p in ps], axis=1)
-    return Posterior(th, np.concatenate([p.summed_log_prior for p in ps]),
-                     np.concatenate([p.log_likelihoods for p in ps]), ps[0].nmc, ps[0].thin,
-                     th.shape[1], ps[0].pnames)
+    """R/parallel_helper.R:59-80: the ncore single-level runs as one posterior
+    whose chains are the runs and whose nmc is nmc x nchain of one run."""
+    return _rebuild([f for f in fits])
+
+
+def RebuildHyper(fits):
+    """R/parallel_helper.R:84-106: the population-level posteriors of ncore
+    hierarchical runs (each fit holds 'phi'), rebuilt as RebuildPosterior."""
+    return _rebuild([f["phi"] for f in fits])
+
+
+def RebuildPosteriors(fits):
+    """R/parallel_helper.R:162-208: one rebuilt posterior per subject of ncore
+    hierarchical runs (each fit holds 'subject_theta')."""
+    if not isinstance(fits, (list, tuple)) or len(fits) == 0:
+        raise ValueError("Input must be a non-empty list of model fits")
+    nsub = len(fits[0]["subject_theta"])
+    return [_rebuild([f["subject_theta"][i] for f in fits]) for i in range(nsub)]

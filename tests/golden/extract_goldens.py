@@ -18,8 +18,8 @@ import sys
 
 import numpy as np
 
-sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
-from rdata import load_rda  # noqa: E402
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))))
+from ggdmc_amd.rdata import load_rda  # noqa: E402
 
 REF_DATA = "/root/reference/tests/testthat/Group1/data"
 OUT_DIR = os.path.dirname(os.path.abspath(__file__))

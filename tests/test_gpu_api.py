@@ -56,7 +56,7 @@ def test_start_hierarchical_and_hyper(gpu):
     again = sampling.RestartSampling(fits, nmc=5)
     assert np.array_equal(again[1]["phi"].theta[:, :, 0], fits[1]["phi"].theta[:, :, -1])
     data = np.stack([s[0][:, 0] for s in job.subject_starts])
-    hyper = sampling.StartSampling_hyper(data, priors, nmc=15, nchain=20, ncore=2, seed=5)
+    hyper = sampling.StartSampling_hyper(data, dmis, priors, nmc=15, nchain=20, ncore=2, seed=5)
     assert hyper[0].theta.shape == (26, 20, 15)
     assert np.isfinite(hyper[0].log_likelihoods).all()
 

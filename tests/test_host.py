@@ -263,3 +263,83 @@ def test_replicated_workload_shares_data_with_new_keys():
     assert all(j.subjects is w.jobs[0].subjects and j.phi_start is w.jobs[0].phi_start for j in r.jobs)
     assert r.n_subjects == 3 * w.n_subjects and r.n_trials == 3 * w.n_trials
     assert workloads.replicate(w, 1) is w
+
+
+# ------------------------------------------- posterior assembly, initialisers
+def _post(run, npar=3, nchain=4, nmc=5):
+    """A posterior whose every entry names its (run, parameter, chain, column)."""
+    from ggdmc_amd.runs import Posterior
+    p, c, m = np.meshgrid(np.arange(npar), np.arange(nchain), np.arange(nmc), indexing="ij")
+    th = run * 10000 + p * 1000 + c * 100 + m
+    lp = -(run * 10000 + c[0] * 100 + m[0]).astype(float)
+    ll = run * 10000 + c[0] * 100 + m[0] + 0.5
+    return Posterior(th.astype(float), lp, ll, nmc, 2, nchain, ["a", "b", "c"], 1)
+
+
+def test_rebuild_posterior_follows_the_reference_layout():
+    # R/parallel_helper.R:15-80: run i becomes chain i; its [npar, nchain, nmc]
+    # theta is flattened column-major (chain fastest) into nchain*nmc columns;
+    # lp / ll matrices are transposed then flattened (column fastest)
+    from ggdmc_amd import sampling
+    runs = [_post(i) for i in range(3)]
+    r = sampling.RebuildPosterior(runs)
+    assert r.theta.shape == (3, 3, 20) and r.nchain == 3 and r.nmc == 20 and r.thin == 2
+    for i in range(3):
+        for p in range(3):
+            for c in range(4):
+                for m in range(5):
+                    assert r.theta[p, i, c + 4 * m] == i * 10000 + p * 1000 + c * 100 + m
+                    assert r.summed_log_prior[i, m + 5 * c] == -(i * 10000 + c * 100 + m)
+                    assert r.log_likelihoods[i, m + 5 * c] == i * 10000 + c * 100 + m + 0.5
+    h = sampling.RebuildHyper([{"phi": x} for x in runs])
+    assert np.array_equal(h.theta, r.theta) and np.array_equal(h.log_likelihoods, r.log_likelihoods)
+    subs = sampling.RebuildPosteriors([{"phi": None, "subject_theta": [_post(i), _post(i + 5)]}
+                                       for i in range(3)])
+    assert len(subs) == 2
+    assert np.array_equal(subs[0].theta, r.theta)
+    assert subs[1].theta[0, 2, 0] == (2 + 5) * 10000
+    with pytest.raises(ValueError):
+        sampling.RebuildPosteriors([])
+
+
+def _cpu_loglike(model, data, thetas, precision=None):
+    om = O.Model(model.winner, model.par_src, model.par_const, model.npar)
+    return np.array([O.lba_loglike(om, data.rt, data.cell_off, t) for t in np.atleast_2d(thetas)])
+
+
+def test_initialise_phi_keeps_the_subject_prior_density(monkeypatch):
+    # R/phi.R:256-332: subject starts come from initialise_theta under p_prior
+    # and keep that lp; phi chains are the first h_prior draws with finite
+    # log prior and hyper likelihood against chain c's subject starts
+    from ggdmc_amd import sampling
+    from ggdmc_amd.priors import sumloghlike, sumlogprior
+    monkeypatch.setattr(sampling, "loglike", _cpu_loglike)
+    fix = fx.load("lba_data6")
+    priors = sampling.set_priors(fix.pop_prior, fix.hyper_prior)
+    dmis = [(fix.model, fix.pop_data(s)) for s in range(2)]
+    (phi, plp, pll), subs = sampling.initialise_phi(4, priors, dmis, seed=11, max_init_attempts=200)
+    for th, lp, ll in subs:
+        for c in range(4):
+            assert lp[c] == sumlogprior(fix.pop_prior, th[:, c])
+    for c in range(4):
+        assert plp[c] == sumlogprior(fix.hyper_prior, phi[:, c])
+        X = np.stack([s[0][:, c] for s in subs])
+        assert pll[c] == sumloghlike(fix.pop_prior, phi[:, c], X)
+        assert np.isfinite(plp[c]) and np.isfinite(pll[c])
+
+
+def test_exhausted_initialisation_raises(monkeypatch):
+    # R/phi.R:322-325: no valid phi after max_init_attempts draws is an error;
+    # StartSampling_hyper initialises through initialise_phi and raises too
+    from ggdmc_amd import sampling
+    monkeypatch.setattr(sampling, "loglike", _cpu_loglike)
+    fix = fx.load("lba_data6")
+    P = fix.P
+    bad = PriorSpec.uniform(2 * P, -2.0, -1.0)          # every scale negative: NaN likelihood
+    priors = sampling.set_priors(fix.pop_prior, bad)
+    dmis = [(fix.model, fix.pop_data(0))]
+    with pytest.raises(RuntimeError, match="Failed to find valid phi after 7 attempts"):
+        sampling.initialise_phi(3, priors, dmis, seed=1, max_init_attempts=7)
+    with pytest.raises(RuntimeError, match="Failed to find valid phi"):
+        sampling.StartSampling_hyper(np.ones((2, P)), dmis, priors, nchain=3, ncore=1, seed=2,
+                                     max_init_attempts=5)
