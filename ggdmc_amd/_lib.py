@@ -107,6 +107,7 @@ def lib():
                             C.POINTER(Samples)]
     L.ggdmc_loglike.argtypes = [C.POINTER(Model), C.POINTER(Subject), _d, C.c_int32, C.c_int32,
                                 _d]
+    L.ggdmc_loglike_noalloc.argtypes = L.ggdmc_loglike.argtypes
     L.ggdmc_session_create.argtypes = [C.POINTER(Config), C.POINTER(Model), C.POINTER(Job),
                                        C.c_int32, C.POINTER(C.c_void_p)]
     L.ggdmc_session_advance.argtypes = [C.c_void_p, C.c_int32]

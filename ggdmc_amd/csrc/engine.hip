@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -906,6 +907,42 @@ static int invalid(const char *msg) {
     return GGDMC_EINVAL;
 }
 
+/* Staging array of ggdmc_loglike_noalloc: a fixed device array (every GPU the
+ * library runs on holds its own copy until the process ends), so that the
+ * call needs no device allocation. In host code `scratch` names a host-side
+ * stand-in, not a device address: the device address of each GPU's copy comes
+ * from hipGetSymbolAddress, taken once per device with that device current,
+ * and only that address is used. Calls on one GPU share the array; a
+ * per-device lock serialises them (overlapping calls would race on it). */
+constexpr size_t kStagingBytes = (size_t)16 << 20;
+static __device__ __align__(256) char scratch[kStagingBytes];
+
+struct Staging {
+    std::mutex mu;       /* held for a whole call */
+    char *ptr = nullptr; /* this device's copy of scratch */
+};
+
+static Staging &staging_for_current_device() {
+    static std::mutex table_mu;
+    static std::vector<std::unique_ptr<Staging>> table;
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(table_mu);
+    if ((int)table.size() <= dev) table.resize(dev + 1);
+    if (!table[dev]) {
+        auto st = std::make_unique<Staging>();
+        void *p = nullptr;
+        HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        require(p != nullptr, "staging array has no device address");
+        st->ptr = static_cast<char *>(p);
+        table[dev] = std::move(st);
+    }
+    return *table[dev];
+}
+
+static int loglike_impl(const ggdmc_model *model, const ggdmc_subject *data, const double *theta,
+                        int32_t n, int32_t precision, double *ll_out, bool noalloc);
+
 /* ================================================================= C ABI */
 extern "C" {
 
@@ -1145,11 +1182,27 @@ int ggdmc_run(const ggdmc_config *cfg, const ggdmc_model *model, const ggdmc_pri
 
 int ggdmc_loglike(const ggdmc_model *model, const ggdmc_subject *data, const double *theta, int32_t n,
                   int32_t precision, double *ll_out) {
+    return loglike_impl(model, data, theta, n, precision, ll_out, false);
+}
+
+int ggdmc_loglike_noalloc(const ggdmc_model *model, const ggdmc_subject *data, const double *theta,
+                          int32_t n, int32_t precision, double *ll_out) {
+    return loglike_impl(model, data, theta, n, precision, ll_out, true);
+}
+
+} // extern "C"
+
+/* ggdmc_loglike and its allocation-free variant: the design, trials, thetas
+ * and output are packed into one device buffer, allocated per call or (noalloc)
+ * the device's staging array, the only difference between the two. */
+static int loglike_impl(const ggdmc_model *model, const ggdmc_subject *data, const double *theta,
+                        int32_t n, int32_t precision, double *ll_out, bool noalloc) {
     if (ggdmc_device_count() < 1) {
         g_error = "no gfx950 device visible";
         return GGDMC_ENODEV;
     }
     void *dbuf = nullptr;
+    std::unique_lock<std::mutex> staging_lock;
     try {
         require(n >= 1 && theta && ll_out, "theta / ll_out missing");
         require(precision == GGDMC_FP64_PLAIN || precision == GGDMC_MIXED_BATCHED,
@@ -1200,8 +1253,19 @@ int ggdmc_loglike(const ggdmc_model *model, const ggdmc_subject *data, const dou
         std::memcpy(&h[o_off], soff, sizeof(soff));
         fill_exp2_table(reinterpret_cast<double *>(&h[o_tab]));
         std::memcpy(&h[o_th], theta, sizeof(double) * (size_t)n * P);
-        HIPCHECK(hipMalloc(&dbuf, total));
-        char *b = static_cast<char *>(dbuf);
+        char *b = nullptr;
+        if (noalloc) {
+            require(total <= kStagingBytes,
+                    "ggdmc_loglike_noalloc: " + std::to_string(total) + " bytes of design, trials, "
+                    "thetas and output exceed the " + std::to_string(kStagingBytes) +
+                    "-byte staging array (use ggdmc_loglike)");
+            Staging &st = staging_for_current_device();
+            staging_lock = std::unique_lock<std::mutex>(st.mu);
+            b = st.ptr;
+        } else {
+            HIPCHECK(hipMalloc(&dbuf, total));
+            b = static_cast<char *>(dbuf);
+        }
         HIPCHECK(hipMemcpy(b, h.data(), o_out, hipMemcpyHostToDevice));
         Dev d{};
         d.P = P;
@@ -1221,7 +1285,7 @@ int ggdmc_loglike(const ggdmc_model *model, const ggdmc_subject *data, const dou
                                                precision, cmax, lds, nullptr,
                                                reinterpret_cast<double *>(b + o_out)));
         HIPCHECK(hipMemcpy(ll_out, b + o_out, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost));
-        HIPCHECK(hipFree(dbuf));
+        if (dbuf) HIPCHECK(hipFree(dbuf));
         dbuf = nullptr;
     } catch (const std::exception &e) {
         if (dbuf) (void)hipFree(dbuf);
@@ -1229,5 +1293,3 @@ int ggdmc_loglike(const ggdmc_model *model, const ggdmc_subject *data, const dou
     }
     return GGDMC_OK;
 }
-
-} // extern "C"

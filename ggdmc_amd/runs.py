@@ -316,13 +316,15 @@ def run(cfg, model, p_prior, h_prior, subjects, phi_start, subject_starts, seed)
     return {"phi": r["phi"], "subject_theta": r["subjects"]}
 
 
-def loglike(model, data, thetas, precision=_lib.MIXED_BATCHED):
-    """sumloglike of parameter vectors thetas [n, P] for one subject."""
+def loglike(model, data, thetas, precision=_lib.MIXED_BATCHED, noalloc=False):
+    """sumloglike of parameter vectors thetas [n, P] for one subject
+    (ggdmc_loglike; noalloc: ggdmc_loglike_noalloc, staged in the library's
+    fixed device array instead of a per-call allocation)."""
     k = Keep()
     cm = model.c_struct(k)
     cs = data.c_struct(k)
     th = k.arr(np.atleast_2d(thetas), np.float64)
     out = np.empty(th.shape[0])
-    check(_lib.lib().ggdmc_loglike(C.byref(cm), C.byref(cs), ptr(th), th.shape[0], int(precision),
-                                   ptr(out)))
+    fn = _lib.lib().ggdmc_loglike_noalloc if noalloc else _lib.lib().ggdmc_loglike
+    check(fn(C.byref(cm), C.byref(cs), ptr(th), th.shape[0], int(precision), ptr(out)))
     return out

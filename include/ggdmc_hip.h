@@ -150,6 +150,17 @@ int ggdmc_run_batch(const ggdmc_config *cfg, const ggdmc_model *model, const ggd
 int ggdmc_loglike(const ggdmc_model *model, const ggdmc_subject *data, const double *theta,
                   int32_t n, int32_t precision, double *ll_out);
 
+/* No counterpart in the reference. ggdmc_loglike for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): same arguments and results (bit for bit),
+ * but the design, trials, thetas and output are staged in a fixed 16 MiB
+ * device array per GPU instead of a per-call allocation; it calls no
+ * hipMalloc / hipFree. Calls whose staging exceeds 16 MiB (about 1.5 M trials)
+ * fail with GGDMC_EINVAL. Calls on one GPU share the array and are serialised
+ * by the library. */
+int ggdmc_loglike_noalloc(const ggdmc_model *model, const ggdmc_subject *data, const double *theta,
+                          int32_t n, int32_t precision, double *ll_out);
+
 /* Sessions: a batch whose inputs and state stay resident in HBM, advanced in
  * steps (restart without re-upload; timing of the device-resident loop). */
 typedef struct ggdmc_session ggdmc_session;
