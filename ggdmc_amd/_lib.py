@@ -41,7 +41,16 @@ class Config(C.Structure):
                 ("is_hblocked", C.c_int32), ("is_pblocked", C.c_int32),
                 ("precision", C.c_int32), ("sweep_groups", C.c_int32),
                 ("trials_per_tile", C.c_int32), ("chains_per_item", C.c_int32),
-                ("launch_mode", C.c_int32), ("iter_offset", C.c_uint64)]
+                ("launch_mode", C.c_int32), ("iter_offset", C.c_uint64),
+                ("pop_debug", C.c_int32), ("sub_debug", C.c_int32)]
+
+
+class TraceRecord(C.Structure):
+    _fields_ = [("job", C.c_int32), ("subject", C.c_int32), ("iteration", C.c_uint32),
+                ("sweep", C.c_int32), ("kind", C.c_int32), ("step", C.c_int32),
+                ("chain", C.c_int32), ("target", C.c_int32), ("tmp_lp", C.c_double),
+                ("tmp_ll", C.c_double), ("cur_post", C.c_double), ("accepted", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class Model(C.Structure):
@@ -117,10 +126,29 @@ def lib():
                                         C.POINTER(C.c_int32)]
     L.ggdmc_session_trial_evals.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     L.ggdmc_session_decomposition.argtypes = [C.c_void_p, _i32, _i32, _i32, _i32]
+    L.ggdmc_session_trace.argtypes = [C.c_void_p, C.POINTER(C.POINTER(TraceRecord)),
+                                      C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.ggdmc_last_trace.argtypes = [C.POINTER(C.POINTER(TraceRecord)), C.POINTER(C.c_int64),
+                                   C.POINTER(C.c_int64)]
     L.ggdmc_session_destroy.argtypes = [C.c_void_p]
     L.ggdmc_session_destroy.restype = None
     _LIB = L
     return L
+
+
+TRACE_DTYPE = np.dtype([("job", np.int32), ("subject", np.int32), ("iteration", np.uint32),
+                        ("sweep", np.int32), ("kind", np.int32), ("step", np.int32),
+                        ("chain", np.int32), ("target", np.int32), ("tmp_lp", np.float64),
+                        ("tmp_ll", np.float64), ("cur_post", np.float64),
+                        ("accepted", np.int32), ("reserved", np.int32)])
+
+
+def trace_array(ptr_, n):
+    """A copy of n trace records at ptr_ as a numpy structured array."""
+    if n <= 0:
+        return np.zeros(0, TRACE_DTYPE)
+    buf = C.string_at(C.cast(ptr_, C.c_void_p), int(n) * C.sizeof(TraceRecord))
+    return np.frombuffer(buf, TRACE_DTYPE).copy()
 
 
 def check(rc):

@@ -27,6 +27,8 @@
 
 #include <cstdint>
 
+#include "../../include/ggdmc_hip.h"
+
 namespace ggdmc {
 
 constexpr int kBlock = 256;        /* threads per workgroup of the sweep kernels */
@@ -98,6 +100,13 @@ struct Dev {
 
     double *out_theta, *out_lp, *out_ll;
     double *phi_out_theta, *phi_out_lp, *phi_out_ll;
+
+    /* decision trace (debug flags): records appended at trace_n, at most
+     * trace_cap kept; null when no level is traced */
+    ggdmc_trace_record *trace;
+    uint32_t *trace_n;
+    uint32_t trace_cap;
+    int32_t trace_pop, trace_sub;
 };
 
 /* A work item of a subject launch: subject x, chains [c0, c1) of chain group

@@ -31,6 +31,7 @@
 
 #include "../../include/ggdmc_hip.h"
 #include "engine.h"
+#include "fp64_math.h"
 #include "lba_math.h"
 
 using namespace ggdmc;
@@ -61,6 +62,10 @@ hipError_t ggdmc_internal_launch_loglike(const Dev *d, const double *thetas, int
 namespace {
 
 thread_local std::string g_error;
+thread_local std::vector<ggdmc_trace_record> g_trace;   /* of the thread's last one-shot run */
+thread_local int64_t g_trace_dropped = 0;
+
+constexpr uint32_t kTraceCap = 1u << 20;   /* decision records kept per advance */
 
 struct Error : std::runtime_error {
     int code;
@@ -160,6 +165,8 @@ struct ggdmc_session {
     std::vector<int> phi_of_job;
     int64_t max_ntr = 0;
     std::vector<double> hyper_rows_h;
+    std::vector<ggdmc_trace_record> trace;   /* decisions of the last advance, ordered */
+    int64_t trace_dropped = 0;
 
     ~ggdmc_session() {
         if (graph) (void)hipGraphExecDestroy(graph);
@@ -185,6 +192,27 @@ struct ggdmc_session {
         uint32_t f = 0;
         HIPCHECK(hipMemcpy(&f, d.fault, sizeof(f), hipMemcpyDeviceToHost));
         if (f != 0) throw Error(GGDMC_EHIP, "population step not published in time: results invalid");
+    }
+    /* the decisions recorded by the last advance, in the reference's print
+     * order: run, iteration, level (population first), sweep, step */
+    void collect_trace() {
+        trace.clear();
+        trace_dropped = 0;
+        if (!d.trace) return;
+        uint32_t n = 0;
+        HIPCHECK(hipMemcpy(&n, d.trace_n, sizeof(n), hipMemcpyDeviceToHost));
+        const uint32_t kept = std::min(n, d.trace_cap);
+        trace.resize(kept);
+        if (kept)
+            HIPCHECK(hipMemcpy(trace.data(), d.trace, sizeof(ggdmc_trace_record) * kept, hipMemcpyDeviceToHost));
+        trace_dropped = (int64_t)n - (int64_t)kept;
+        std::stable_sort(trace.begin(), trace.end(), [](const ggdmc_trace_record &a, const ggdmc_trace_record &b) {
+            if (a.job != b.job) return a.job < b.job;
+            if (a.iteration != b.iteration) return a.iteration < b.iteration;
+            if (a.subject != b.subject) return a.subject < b.subject;
+            if (a.sweep != b.sweep) return a.sweep < b.sweep;
+            return a.step < b.step;
+        });
     }
     void drop_graph() {
         if (graph) (void)hipGraphExecDestroy(graph);
@@ -439,19 +467,20 @@ void ggdmc_session::build(const ggdmc_config *c, const ggdmc_model *model, const
     d.pr_up = pool.upload(pr_up, stream);
     {
         /* cached logarithms of the fixed priors: unif log(upper - lower);
-         * tnorm log(scale), log(mass on [lower, upper]) with Phi as R's pnorm */
+         * tnorm log(scale), log(mass on [lower, upper]) with Phi as R's pnorm;
+         * the kernels' own functions, so cached and direct terms agree */
         auto pnorm = [](double z) {
-            return z > 0.0 ? 1.0 - 0.5 * std::erfc(z * M_SQRT1_2) : 0.5 * std::erfc(-z * M_SQRT1_2);
+            return z == HUGE_VAL ? 1.0 : (z == -HUGE_VAL ? 0.0 : pnorm_acc(z));
         };
         std::vector<double> sc(2 * std::max<size_t>(pr_dist.size(), 1), 0.0);
         for (size_t k = 0; k < pr_dist.size(); ++k) {
             if (pr_dist[k] == GGDMC_DIST_UNIF) {
-                sc[2 * k] = std::log(pr_p1[k] - pr_p0[k]);
+                sc[2 * k] = log_acc(pr_p1[k] - pr_p0[k]);
             } else {
                 const double a = (pr_lo[k] - pr_p0[k]) / pr_p1[k], b = (pr_up[k] - pr_p0[k]) / pr_p1[k];
                 const double Z = a > 0.0 ? pnorm(-a) - pnorm(-b) : pnorm(b) - pnorm(a);
-                sc[2 * k] = std::log(pr_p1[k]);
-                sc[2 * k + 1] = std::log(Z);
+                sc[2 * k] = log_acc(pr_p1[k]);
+                sc[2 * k + 1] = log_acc(Z);
             }
         }
         d.scache = pool.upload(sc, stream);
@@ -479,6 +508,17 @@ void ggdmc_session::build(const ggdmc_config *c, const ggdmc_model *model, const
     HIPCHECK(hipMemsetAsync(d.phi_ready, 0, sizeof(uint32_t) * std::max(F, 1) * n, stream));
     d.fault = pool.alloc<uint32_t>(1);
     HIPCHECK(hipMemsetAsync(d.fault, 0, sizeof(uint32_t), stream));
+    d.trace = nullptr;
+    d.trace_n = nullptr;
+    d.trace_cap = 0;
+    d.trace_pop = cfg.pop_debug != 0;
+    d.trace_sub = cfg.sub_debug != 0;
+    if (d.trace_pop || d.trace_sub) {
+        d.trace = pool.alloc<ggdmc_trace_record>(kTraceCap);
+        d.trace_n = pool.alloc<uint32_t>(1);
+        d.trace_cap = kTraceCap;
+        HIPCHECK(hipMemsetAsync(d.trace_n, 0, sizeof(uint32_t), stream));
+    }
     d.evals = pool.alloc<unsigned long long>((size_t)std::max(S, 1));
     HIPCHECK(hipMemsetAsync(d.evals, 0, sizeof(unsigned long long) * std::max(S, 1), stream));
     {
@@ -998,6 +1038,7 @@ int ggdmc_session_advance(ggdmc_session *s, int32_t n_iter) {
     try {
         const uint32_t left = s->next_iter <= s->last_iter ? s->last_iter - s->next_iter + 1 : 0;
         const uint32_t n = std::min<uint32_t>((uint32_t)n_iter, left);
+        if (s->d.trace) HIPCHECK(hipMemsetAsync(s->d.trace_n, 0, sizeof(uint32_t), s->stream));
         HIPCHECK(hipEventRecord(s->ev0, s->stream));
         if (s->mode == GGDMC_FP64_PLAIN) s->run_plain(n);
         else s->run_batched(n);
@@ -1005,6 +1046,7 @@ int ggdmc_session_advance(ggdmc_session *s, int32_t n_iter) {
         HIPCHECK(hipEventSynchronize(s->ev1));
         HIPCHECK(hipEventElapsedTime(&s->last_ms, s->ev0, s->ev1));
         s->check_fault();
+        s->collect_trace();
     } catch (const std::exception &e) {
         return fail(e);
     }
@@ -1122,13 +1164,34 @@ int ggdmc_session_download(ggdmc_session *s) {
 
 void ggdmc_session_destroy(ggdmc_session *s) { delete s; }
 
+int ggdmc_session_trace(ggdmc_session *s, const ggdmc_trace_record **records, int64_t *n,
+                        int64_t *dropped) {
+    if (!s || !records || !n) return invalid("session / records / n is null");
+    *records = s->trace.empty() ? nullptr : s->trace.data();
+    *n = (int64_t)s->trace.size();
+    if (dropped) *dropped = s->trace_dropped;
+    return GGDMC_OK;
+}
+
+int ggdmc_last_trace(const ggdmc_trace_record **records, int64_t *n, int64_t *dropped) {
+    if (!records || !n) return invalid("records / n is null");
+    *records = g_trace.empty() ? nullptr : g_trace.data();
+    *n = (int64_t)g_trace.size();
+    if (dropped) *dropped = g_trace_dropped;
+    return GGDMC_OK;
+}
+
 int ggdmc_run_batch(const ggdmc_config *cfg, const ggdmc_model *model, const ggdmc_job *jobs,
                     int32_t njobs) {
     ggdmc_session *s = nullptr;
+    g_trace.clear();
+    g_trace_dropped = 0;
     int rc = ggdmc_session_create(cfg, model, jobs, njobs, &s);
     if (rc != GGDMC_OK) return rc;
     rc = ggdmc_session_advance(s, (int32_t)s->last_iter);
     if (rc == GGDMC_OK) rc = ggdmc_session_download(s);
+    g_trace.swap(s->trace);
+    g_trace_dropped = s->trace_dropped;
     ggdmc_session_destroy(s);
     return rc;
 }

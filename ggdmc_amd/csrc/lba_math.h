@@ -288,7 +288,7 @@ GGDMC_FN double naive_trial_term(const AccK *kc, int n_acc, int w, double rt) {
         const double F = cdf != cdf ? cdf : (cdf > kFloor ? cdf : kFloor);
         dens *= (1.0 - F);
     }
-    return std::log(dens <= 0.0 ? 2.220446049250313e-16 : dens);
+    return Ops::log(dens <= 0.0 ? 2.220446049250313e-16 : dens);
 }
 
 /* Batched-path trial term, mixed evaluation; ill is set when any
@@ -358,7 +358,7 @@ GGDMC_FN double mixed_trial_term(const AccK *kc, int n_acc, int w, double rt) {
 /* ----------------------------------------------------------- per (cell, acc)
  * Resolves the six families from theta / constants and fills AccK.
  * par_src / par_const are [n_acc * 6] for the cell. Returns false if st0 != 0
- * (the path implements st0 = 0 only). Phi(x) = 0.5*erfc(-x/sqrt(2)).
+ * (the path implements st0 = 0 only).
  */
 template <class Ops>
 GGDMC_FN bool make_acck(const double *theta, const int32_t *src6, const double *const6,
@@ -373,8 +373,8 @@ GGDMC_FN bool make_acck(const double *theta, const int32_t *src6, const double *
     k.t0 = f[5];
     k.invA = 1.0 / k.A;
     const double x = k.v / k.s;
-    /* one erfc gives both tails: q = Phi(-|x|) exactly, the other is 1 - q */
-    const double q = 0.5 * Ops::erfc(std::fabs(x) * kSqrt1_2);
+    /* one tail gives both: q = Phi(-|x|), the other is 1 - q */
+    const double q = Ops::qtail(std::fabs(x));
     const double Pv = x >= 0.0 ? 1.0 - q : (x == x ? q : x);   /* Phi(v/s)  */
     k.Pc = x >= 0.0 ? q : (x == x ? 1.0 - q : x);              /* Phi(-v/s) */
     k.den = Pv > kFloor ? Pv : (Pv == Pv ? kFloor : Pv);

@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include "engine.h"
+#include "fp64_math.h"
 #include "lba_math.h"
 #include "philox.h"
 #include "../../include/ggdmc_hip.h"
@@ -44,14 +45,12 @@ struct DevOps {
     static __device__ __forceinline__ float exp2(float x) { return __builtin_amdgcn_exp2f(x); }
     static __device__ __forceinline__ float log2(float x) { return __builtin_amdgcn_logf(x); }
     static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
-    static __device__ __forceinline__ double erfc(double x) { return ::erfc(x); }
+    /* the library's own FP64 functions (fp64_math.h), no math-library code */
+    static __device__ __forceinline__ double qtail(double a) { return qtail_acc(a); }
     /* R's pnorm: the lower tail accurately, above 0 as 1 - upper tail */
-    static __device__ __forceinline__ double pnorm(double z) {
-        return z > 0.0 ? 1.0 - 0.5 * ::erfc(z * kSqrt1_2) : 0.5 * ::erfc(-z * kSqrt1_2);
-    }
-    static __device__ __forceinline__ double dnorm(double z) {
-        return kInvSqrt2Pi * ::exp(-0.5 * z * z);
-    }
+    static __device__ __forceinline__ double pnorm(double z) { return pnorm_acc(z); }
+    static __device__ __forceinline__ double dnorm(double z) { return kInvSqrt2Pi * exp_acc(-0.5 * z * z); }
+    static __device__ __forceinline__ double log(double x) { return log_acc(x); }
 };
 
 enum { MODE_MIXED = 0, MODE_NAIVE = 1 };
@@ -113,47 +112,93 @@ struct TrialView {
     int64_t origin;
 };
 
+/* The terms a lane adds to its chunk sum: trials i1 and i1 + 64 of the
+ * chunk (zero past t_end). MODE_MIXED: ill1 / ill2 report trials whose mixed
+ * evaluation is ill-conditioned, and fix: those trials are evaluated with the
+ * reference formula instead (the fix-up pass). */
+template <int MODE, bool FIX>
+__device__ __forceinline__ void lane_terms(const Dev &d, const TrialView &tv, const AccK *kc, int64_t base, int64_t t_end,
+                                           double &a, double &b, bool &ill1, bool &ill2) {
+    const int lane = threadIdx.x & 63;
+    const int64_t i1 = base + lane, i2 = i1 + 64;
+    const bool h1 = i1 < t_end, h2 = i2 < t_end;
+    /* lanes past the end evaluate the chunk's first trial and drop it, so
+     * that both evaluations are straight-line code */
+    const int64_t r1 = (h1 ? i1 : base) - tv.origin;
+    const int64_t r2 = (h2 ? i2 : base) - tv.origin;
+    const int code1 = tv.code[r1], code2 = tv.code[r2];
+    const double rt1 = tv.rt[r1], rt2 = tv.rt[r2];
+    const AccK *k1 = kc + (code1 >> 3) * d.n_acc, *k2 = kc + (code2 >> 3) * d.n_acc;
+    a = b = 0.0;
+    ill1 = ill2 = false;
+    if (MODE == MODE_MIXED) {
+        double ma, mb;
+        mixed_trial_term_flagged2<DevOps>(k1, code1 & 7, rt1, k2, code2 & 7, rt2, d.n_acc, ma, mb, ill1, ill2);
+        ill1 = h1 && ill1;
+        ill2 = h2 && ill2;
+        a = h1 ? ma : 0.0;
+        b = h2 ? mb : 0.0;
+        if constexpr (FIX) {
+            if (ill1) a = naive_trial_term<DevOps>(k1, d.n_acc, code1 & 7, rt1);
+            if (ill2) b = naive_trial_term<DevOps>(k2, d.n_acc, code2 & 7, rt2);
+        }
+    } else {
+        if (h1) a = naive_trial_term<DevOps>(k1, d.n_acc, code1 & 7, rt1);
+        if (h2) b = naive_trial_term<DevOps>(k2, d.n_acc, code2 & 7, rt2);
+    }
+}
+
+/* Canonical chunk sums (see engine.h): C[c * c_stride + jj] for chains c < kb
+ * and chunks j_lo + jj < j_lo + nj of the trials [t0, t_end). A wave takes one
+ * (chain, chunk) unit at a time; chains with active[c] == 0 get 0.
+ * MODE_MIXED: a unit holding an ill-conditioned trial is marked in pend (LDS
+ * bits, one per unit) and summed by a second pass after the others, which
+ * evaluates its flagged trials with the reference formula: the same per-lane
+ * terms, hence the same bits, as evaluating them in place, without the
+ * reference formula's code (and registers) inside the main loop. */
 template <int MODE>
 __device__ __forceinline__ void chunk_sums(const Dev &d, const TrialView &tv, int64_t t0, int64_t t_end, int j_lo, int nj,
-                           int kb, const int *active, const double *acck, double *C, int c_stride) {
+                                           int kb, const int *active, const double *acck, double *C, int c_stride,
+                                           uint32_t *pend) {
     static_assert(kChunk == 128, "two trials per lane");
     const int lane = threadIdx.x & 63;
     /* the wave index is uniform: keep the unit loop scalar */
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nw = blockDim.x >> 6;
     const int per_chain = d.n_cond * d.n_acc;
-    for (int u = wave; u < kb * nj; u += nw) {
+    const int nu = kb * nj;
+    if (MODE == MODE_MIXED) {
+        for (int w = threadIdx.x; w < (nu + 31) / 32; w += blockDim.x) pend[w] = 0u;
+        __syncthreads();
+    }
+    for (int u = wave; u < nu; u += nw) {
         const int c = u / nj, jj = u - c * nj;
         double v = 0.0;
         if (active[c]) {
             const AccK *kc = reinterpret_cast<const AccK *>(acck) + (size_t)c * per_chain;
-            const int64_t i1 = t0 + (int64_t)(j_lo + jj) * kChunk + lane, i2 = i1 + 64;
-            double a = 0.0, b = 0.0;
-            const bool h1 = i1 < t_end, h2 = i2 < t_end;
-            /* lanes past the end evaluate the chunk's first trial and drop it,
-             * so that both evaluations are straight-line code */
-            const int64_t r1 = (h1 ? i1 : t0 + (int64_t)(j_lo + jj) * kChunk) - tv.origin;
-            const int64_t r2 = (h2 ? i2 : t0 + (int64_t)(j_lo + jj) * kChunk) - tv.origin;
-            const int code1 = tv.code[r1], code2 = tv.code[r2];
-            const double rt1 = tv.rt[r1], rt2 = tv.rt[r2];
-            const AccK *k1 = kc + (code1 >> 3) * d.n_acc, *k2 = kc + (code2 >> 3) * d.n_acc;
-            if (MODE == MODE_MIXED) {
-                bool ill1 = false, ill2 = false;
-                double ma, mb;
-                mixed_trial_term_flagged2<DevOps>(k1, code1 & 7, rt1, k2, code2 & 7, rt2, d.n_acc, ma, mb, ill1,
-                                                  ill2);
-                a = h1 ? ma : 0.0;
-                b = h2 ? mb : 0.0;
-                /* rare: ill-conditioned trials take the reference formula */
-                if (__builtin_expect(h1 && ill1, 0)) a = naive_trial_term<DevOps>(k1, d.n_acc, code1 & 7, rt1);
-                if (__builtin_expect(h2 && ill2, 0)) b = naive_trial_term<DevOps>(k2, d.n_acc, code2 & 7, rt2);
-            } else {
-                if (h1) a = naive_trial_term<DevOps>(k1, d.n_acc, code1 & 7, rt1);
-                if (h2) b = naive_trial_term<DevOps>(k2, d.n_acc, code2 & 7, rt2);
+            double a, b;
+            bool ill1, ill2;
+            lane_terms<MODE, false>(d, tv, kc, t0 + (int64_t)(j_lo + jj) * kChunk, t_end, a, b, ill1, ill2);
+            if (MODE == MODE_MIXED && __builtin_expect(__builtin_amdgcn_ballot_w64(ill1 || ill2) != 0, 0)) {
+                if (lane == 0) atomicOr(pend + (u >> 5), 1u << (u & 31));
+                continue;
             }
             v = a + b;
         }
         v = wave_sum(v);
         if (lane == 0) C[(size_t)c * c_stride + jj] = v;
+    }
+    if (MODE == MODE_MIXED) {
+        __syncthreads();
+        for (int u = wave; u < nu; u += nw) {
+            if (__builtin_expect(((pend[u >> 5] >> (u & 31)) & 1u) == 0, 1)) continue;
+            const int c = u / nj, jj = u - c * nj;
+            const AccK *kc = reinterpret_cast<const AccK *>(acck) + (size_t)c * per_chain;
+            double a, b;
+            bool ill1, ill2;
+            lane_terms<MODE, true>(d, tv, kc, t0 + (int64_t)(j_lo + jj) * kChunk, t_end, a, b, ill1, ill2);
+            const double v = wave_sum(a + b);
+            if (lane == 0) C[(size_t)c * c_stride + jj] = v;
+        }
     }
 }
 
@@ -172,14 +217,14 @@ __device__ double log_prior1(int dist, double p0, double p1, double lo, double u
     if (x != x) return x;
     if (dist == GGDMC_DIST_UNIF) {
         if (x < p0 || x > p1) return -__builtin_inf();
-        return -::log(p1 - p0);
+        return -log_acc(p1 - p0);
     }
     if (dist == GGDMC_DIST_TNORM) {
         if (!(p1 > 0.0)) return __builtin_nan("");
         if (x < lo || x > up) return -__builtin_inf();
         const double z = (x - p0) / p1;
         const double Z = tnorm_mass((lo - p0) / p1, (up - p0) / p1);
-        return -0.5 * z * z - 0.918938533204672741780329736406 - ::log(p1) - ::log(Z);
+        return -0.5 * z * z - 0.918938533204672741780329736406 - log_acc(p1) - log_acc(Z);
     }
     return __builtin_nan("");
 }
@@ -237,9 +282,44 @@ __device__ double subject_logprior(const Dev &d, const JobDev &job, int chain, c
 }
 
 __device__ __forceinline__ bool mh_accept(double tpost, double cur, double U) {
-    const double ratio = ::exp(tpost - cur);
+    const double ratio = exp_acc(tpost - cur);
     if (ratio != ratio) return false;
     return U < ratio;
+}
+
+/* ------------------------------------------------------------ decision trace
+ * With a debug flag set, every Metropolis decision of a traced level is
+ * appended to d.trace (at most d.trace_cap records); the host orders them. */
+__device__ __forceinline__ bool subject_traced(const Dev &d, int x) {
+    return d.trace != nullptr && d.trace_sub &&
+           (d.subj_jobs[x].kind != GGDMC_KIND_HIER || d.subj_unit[x] <= 2);
+}
+
+__device__ __forceinline__ bool phi_traced(const Dev &d, const JobDev &job) {
+    return d.trace != nullptr && (job.kind == GGDMC_KIND_HIER ? d.trace_pop : d.trace_sub);
+}
+
+__device__ __forceinline__ void trace_put(const Dev &d, int job, int subject, const Stream &st, int kind,
+                                       int step, int chain, int target, double tlp, double tll,
+                                       double cur, bool acc) {
+    const uint32_t k = atomicAdd(d.trace_n, 1u);
+    if (k >= d.trace_cap) return;
+    const double diff = tlp + tll - cur;
+    ggdmc_trace_record r;
+    r.job = job;
+    r.subject = subject;
+    r.iteration = st.iter - d.philox_offset;
+    r.sweep = (int32_t)st.sweep;
+    r.kind = kind;
+    r.step = step;
+    r.chain = chain;
+    r.target = target;
+    r.tmp_lp = tlp;
+    r.tmp_ll = tll;
+    r.cur_post = cur;
+    r.accepted = acc ? 1 : (diff != diff ? -1 : 0);
+    r.reserved = 0;
+    d.trace[k] = r;
 }
 
 /* -------------------------------------------------------- LDS carve-up */
@@ -258,6 +338,7 @@ struct SweepLds {
     int *sub;               /* [n] migration subset */
     int *pdist;             /* [P] prior distribution of each parameter */
     int *flag;              /* [4] */
+    uint32_t *pend;         /* [(kmax*cmax + 31)/32] units of a chunk-sum batch to fix up */
     double *pcst;           /* [per_chain*6] design constants (par_const) */
     int32_t *psrc;          /* [per_chain*6] design sources (par_src) */
 };
@@ -267,7 +348,7 @@ __host__ __device__ inline size_t sweep_lds_bytes(int n, int P, int kmax, int pe
     const size_t dbl = (size_t)n * P + 2 * (size_t)n + 7 * (size_t)kmax * P + 2 * (size_t)P +
                        3 * (size_t)kmax + (size_t)chain_batch * per_chain * kAccKDoubles +
                        (size_t)kmax * cmax;
-    const size_t ints = 4 * (size_t)kmax + n + P + 4 + (size_t)per_chain * 6;
+    const size_t ints = 4 * (size_t)kmax + n + P + 4 + (size_t)per_chain * 6 + ((size_t)kmax * cmax + 31) / 32;
     return (dbl + (size_t)tstage + (size_t)per_chain * 6) * sizeof(double) + ints * sizeof(int) +
            (((size_t)tstage * sizeof(uint16_t) + 7) & ~(size_t)7);
 }
@@ -301,7 +382,8 @@ __device__ __forceinline__ SweepLds carve(char *base, int n, int P, int kmax, in
     L.sub = q; q += n;
     L.pdist = q; q += P;
     L.flag = q; q += 4;
-    L.psrc = q;
+    L.psrc = q; q += per_chain * 6;
+    L.pend = reinterpret_cast<uint32_t *>(q);
     return L;
 }
 
@@ -516,7 +598,7 @@ __device__ __forceinline__ void proposals_loglike(const Dev &d, const SweepLds &
         __syncthreads();
         for (int jb = 0; jb < nj; jb += cmax) {
             const int nbat = min(cmax, nj - jb);
-            chunk_sums<MODE>(d, tv, t0, t_end, j_lo + jb, nbat, kb, L.active + cb, L.acck, L.csum, cmax);
+            chunk_sums<MODE>(d, tv, t0, t_end, j_lo + jb, nbat, kb, L.active + cb, L.acck, L.csum, cmax, L.pend);
             __syncthreads();
             if (to_global) {
                 for (int e = threadIdx.x; e < kb * nbat; e += blockDim.x) {
@@ -538,12 +620,16 @@ __device__ __forceinline__ void proposals_loglike(const Dev &d, const SweepLds &
 
 /* Metropolis accept of proposals c < k for chains i0 + c (src/de.cpp:81-108):
  * state updated in LDS and in global memory. */
-__device__ __forceinline__ void accept_proposals(const Dev &d, const SweepLds &L, int x, int i0, int k) {
+__device__ __forceinline__ void accept_proposals(const Dev &d, const SweepLds &L, int x, int i0, int k,
+                                                 const Stream &st) {
     const int n = d.nchain, P = d.P;
+    const bool traced = subject_traced(d, x);
     for (int c = threadIdx.x; c < k; c += blockDim.x) {
         const int i = i0 + c;
         const double cur = L.lp[i] + L.ll[i];
         const bool a = mh_accept(L.tlp[c] + L.tll[c], cur, L.U[c]);
+        if (__builtin_expect(traced, 0))
+            trace_put(d, d.subj_job[x], d.subj_unit[x], st, 0, i, i, i, L.tlp[c], L.tll[c], cur, a);
         L.acc[c] = a;
         if (a) {
             L.lp[i] = L.tlp[c];
@@ -606,6 +692,8 @@ __device__ __forceinline__ void subject_migration(const Dev &d, const SweepLds &
             if (L.active[0]) atomicAdd(d.evals + x, (unsigned long long)(t1 - t0));
             const double cur = L.lp[dst] + L.ll[dst];
             const bool a = mh_accept(L.tlp[0] + L.tll[0], cur, L.U[0]);
+            if (__builtin_expect(subject_traced(d, x), 0))
+                trace_put(d, d.subj_job[x], d.subj_unit[x], st, 1, k, src, dst, L.tlp[0], L.tll[0], cur, a);
             L.flag[1] = a;
             if (a) {
                 L.lp[dst] = L.tlp[0];
@@ -779,8 +867,8 @@ __device__ __forceinline__ double hll_term(const PhiLds &L, int p, double mu, do
 __device__ __forceinline__ void phi_log_consts(const PhiLds &L, int p, double mu, double sg,
                                                double *lc) {
     const double lo = L.sb[2 * p], up = L.sb[2 * p + 1];
-    lc[0] = ::log(sg);
-    lc[1] = ::log(tnorm_mass((lo - mu) / sg, (up - mu) / sg));
+    lc[0] = log_acc(sg);
+    lc[1] = log_acc(tnorm_mass((lo - mu) / sg, (up - mu) / sg));
 }
 
 __device__ __forceinline__ double hyper_prior_term(const Dev &d, const JobDev &job, int q, double x) {
@@ -986,6 +1074,8 @@ __device__ __forceinline__ void phi_migration(const Dev &d, const JobDev &job, c
                 const double cur = L.plp[dst] + L.pll[dst];
                 const U4 b0 = sd.block(slot, 0);
                 const bool a = mh_accept(tlp + tll, cur, u53(b0.x[0], b0.x[1]));
+                if (__builtin_expect(phi_traced(d, job), 0))
+                    trace_put(d, d.phi_job[job.phi_index], -1, sd, 1, k, src, dst, tlp, tll, cur, a);
                 L.flag[1] = a;
                 if (a) {
                     L.plp[dst] = tlp;
@@ -1074,6 +1164,8 @@ __device__ __forceinline__ void phi_crossover(const Dev &d, const JobDev &job, c
             if (hier) L.pll[i] = ref;
             const double cur = L.pll[i] + L.plp[i];
             const bool a = mh_accept(tlp + tll, cur, L.U[c]);
+            if (__builtin_expect(phi_traced(d, job), 0))
+                trace_put(d, d.phi_job[job.phi_index], -1, st, 0, i, i, i, tlp, tll, cur, a);
             L.acc[c] = a;
             if (a) {
                 L.plp[i] = tlp;
@@ -1409,6 +1501,8 @@ __device__ __forceinline__ void phi_chain(const Dev &d, int f, int i, uint32_t i
         if (t == 0) {
             const double cur = ref + misc[0];
             const bool a = mh_accept(tlp + tll, cur, U);
+            if (__builtin_expect(phi_traced(d, job), 0))
+                trace_put(d, d.phi_job[f], -1, st, 0, i, i, i, tlp, tll, cur, a);
             flag[1] = a;
             misc[2] = a ? tlp : misc[0];
             misc[3] = a ? tll : ref;
@@ -1594,7 +1688,7 @@ __device__ __forceinline__ void late_prior(const Dev &d, const JobDev &job, cons
 
 /* ------------------------------------------------------------ sweep kernel */
 template <int MODE, bool MULTI>
-__global__ __launch_bounds__(kBlock, 2) void subject_sweep_kernel(Dev d, SweepArgs a) {
+__global__ __launch_bounds__(kBlock, 3) void subject_sweep_kernel(Dev d, SweepArgs a) {
     extern __shared__ __attribute__((aligned(16))) char lds_raw[];
     /* several iterations per launch (MULTI): segment seg of seg_wgs
      * workgroups runs iteration base + seg; its workgroups wait only on
@@ -1864,7 +1958,7 @@ __global__ __launch_bounds__(kBlock, 2) void subject_sweep_kernel(Dev d, SweepAr
                 wait_chains(d, job.phi_index, i0, k, iter, false);
                 late_prior(d, job, L, i0, k);
             }
-            accept_proposals(d, L, x, i0, k);
+            accept_proposals(d, L, x, i0, k, st);
             if (a.chained) publish_item(ready + (size_t)g * a.rmax, rng, iter);
         }
     }
@@ -1910,6 +2004,7 @@ __global__ __launch_bounds__(kBlock) void plain_subject_kernel(Dev d, uint32_t i
                 misc[1] = subject_logprior(d, job, src, prop);
                 info[0] = 1;
                 info[1] = dst;
+                info[2] = src;
             }
         } else {
             const int para = blocked ? sweep : -1;
@@ -1946,7 +2041,17 @@ __global__ __launch_bounds__(kBlock) void plain_subject_kernel(Dev d, uint32_t i
         const int dst = info[1];
         atomicAdd(d.evals + s, (unsigned long long)(t_hi - t_lo));
         const double cur = d.lp[(size_t)s * n + dst] + d.ll[(size_t)s * n + dst];
-        if (mh_accept(misc[1] + tll, cur, misc[0])) {
+        const bool acc = mh_accept(misc[1] + tll, cur, misc[0]);
+        if (__builtin_expect(subject_traced(d, s), 0)) {
+            /* crossover records carry the sweep of their stream, migration
+             * steps the decision's */
+            const Stream tr = migrate ? sd
+                                      : make_stream(seed, iter + d.philox_offset, (uint32_t)d.subj_unit[s],
+                                                    (uint32_t)sweep);
+            trace_put(d, d.subj_job[s], d.subj_unit[s], tr, migrate ? 1 : 0, j, migrate ? info[2] : j, dst,
+                      misc[1], tll, cur, acc);
+        }
+        if (acc) {
             for (int p = 0; p < P; ++p) th[dst * P + p] = prop[p];
             d.lp[(size_t)s * n + dst] = misc[1];
             d.ll[(size_t)s * n + dst] = tll;
@@ -2057,7 +2162,12 @@ __global__ __launch_bounds__(kBlock) void plain_phi_kernel(Dev d, uint32_t iter,
     const double tll = plain_hyper_ll(d, job, prop, src, rows, red);
     if (threadIdx.x == 0) {
         const double cur = plp[dst] + pll[dst];
-        if (mh_accept(misc[1] + tll, cur, misc[0])) {
+        const bool acc = mh_accept(misc[1] + tll, cur, misc[0]);
+        if (__builtin_expect(phi_traced(d, job), 0)) {
+            const Stream tr = info[0] == 2 ? sd : make_stream(seed, iter + d.philox_offset, unit, (uint32_t)sweep);
+            trace_put(d, d.phi_job[f], -1, tr, info[0] == 2 ? 1 : 0, j, src, dst, misc[1], tll, cur, acc);
+        }
+        if (acc) {
             for (int p = 0; p < P2; ++p) ph[(size_t)dst * P2 + p] = prop[p];
             plp[dst] = misc[1];
             pll[dst] = tll;

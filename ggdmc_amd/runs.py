@@ -148,16 +148,45 @@ class _Out:
 def make_config(nmc, nchain, thin=1, pop_migration_prob=0.0, sub_migration_prob=0.0,
                 gamma_precursor=2.38, rp=0.001, is_hblocked=False, is_pblocked=False,
                 precision=_lib.MIXED_BATCHED, sweep_groups=0, trials_per_tile=0,
-                chains_per_item=0, launch_mode=0, iter_offset=0):
+                chains_per_item=0, launch_mode=0, iter_offset=0, pop_debug=False,
+                sub_debug=False):
     """ggdmc_config. trials_per_tile / chains_per_item / launch_mode 0 let the
     library autotune the work split; launch_mode 1 = a subject launch per chain
     group, 2 = one for every group, 3 = one for every group and 32 iterations.
-    None of them changes a result bit."""
+    None of them changes a result bit. pop_debug / sub_debug record the
+    decisions of the population level / of the subjects (Session.trace)."""
     return _lib.Config(int(nmc), int(thin), int(nchain), float(pop_migration_prob),
                        float(sub_migration_prob), float(gamma_precursor), float(rp),
                        int(bool(is_hblocked)), int(bool(is_pblocked)), int(precision),
                        int(sweep_groups), int(trials_per_tile), int(chains_per_item),
-                       int(launch_mode), int(iter_offset))
+                       int(launch_mode), int(iter_offset), int(bool(pop_debug)),
+                       int(bool(sub_debug)))
+
+
+def format_trace(records):
+    """The decision records as src/de.cpp prints them with debug on
+    (update_theta :85-106, the population steps :441-463 / :533-558): one
+    line per decision, '[C i]: lp, ll, (new vs old posterior), [ratio]-> accepted'
+    or 'rejected' ('rejected (mh nan)' for a NaN ratio), migration steps as
+    '[C src] vs [C dst]'; a header per run, iteration and level."""
+    lines = []
+    last = None
+    for r in records:
+        key = (int(r["job"]), int(r["iteration"]), int(r["subject"]))
+        if key != last:
+            lvl = "population" if r["subject"] < 0 else "subject %d" % r["subject"]
+            lines.append("run %d, iteration %d, %s:" % (key[0], key[1], lvl))
+            last = key
+        who = ("  [C%d]: " % r["target"]) if r["kind"] == 0 else \
+            ("  [C%d] vs [C%d]: " % (r["chain"], r["target"]))
+        if r["accepted"] == 1:
+            post = r["tmp_lp"] + r["tmp_ll"]
+            ratio = np.exp(post - r["cur_post"]) if np.isfinite(post - r["cur_post"]) else np.inf
+            lines.append("%s%.6g, %.6g, (%.6g vs %.6g), \t[%.6g]-> accepted"
+                         % (who, r["tmp_lp"], r["tmp_ll"], post, r["cur_post"], ratio))
+        else:
+            lines.append(who + ("rejected (mh nan)" if r["accepted"] < 0 else "rejected"))
+    return "\n".join(lines)
 
 
 @dataclass
@@ -254,6 +283,15 @@ class Session:
         v = C.c_double()
         check(_lib.lib().ggdmc_session_trial_evals(self.h, C.byref(v)))
         return v.value
+
+    def trace(self):
+        """Decision records of the last advance (pop_debug / sub_debug), in the
+        reference's print order, and how many were dropped beyond the cap."""
+        p = C.POINTER(_lib.TraceRecord)()
+        n = C.c_int64()
+        dr = C.c_int64()
+        check(_lib.lib().ggdmc_session_trace(self.h, C.byref(p), C.byref(n), C.byref(dr)))
+        return _lib.trace_array(p, n.value), dr.value
 
     def decomposition(self):
         """(chains per item or -1 for one item per subject, trials per item or 0

@@ -20,7 +20,7 @@ import numpy as np
 
 from . import _lib
 from .priors import logdprior, rprior, sumloghlike, sumlogprior
-from .runs import JobSpec, PriorSpec, Posterior, Session, loglike, make_config
+from .runs import JobSpec, PriorSpec, Posterior, Session, format_trace, loglike, make_config
 
 
 def set_priors(p_prior, h_prior=None):
@@ -129,13 +129,22 @@ def _config(nmc, nchain, thin, kw):
                        is_pblocked=kw.get("is_pblocked", False),
                        precision=kw.get("precision", _lib.MIXED_BATCHED),
                        sweep_groups=kw.get("sweep_groups", 0),
-                       iter_offset=kw.get("iter_offset", 0))
+                       iter_offset=kw.get("iter_offset", 0),
+                       pop_debug=kw.get("pop_debug", False), sub_debug=kw.get("sub_debug", False))
 
 
 def _execute(kind, model, jobs, cfg):
+    """Runs the jobs to completion; with a debug flag set, prints their
+    decisions as the reference does (src/de.cpp:85-106)."""
     s = Session(cfg, model, jobs)
     try:
         s.advance(s.total_iters)
+        if cfg.pop_debug or cfg.sub_debug:
+            rec, dropped = s.trace()
+            if len(rec):
+                print(format_trace(rec))
+            if dropped:
+                print("(%d further decisions not recorded)" % dropped)
         return s.download()
     finally:
         s.close()
@@ -143,7 +152,7 @@ def _execute(kind, model, jobs, cfg):
 
 def StartSampling_subject(dmi, priors, samples_list=None, nmc=500, nchain=None, thin=1,
                           max_init_attempts=1000, sub_migration_prob=0.0, gamma_precursor=2.38,
-                          rp=0.001, is_pblocked=False, ncore=3, seed=None,
+                          rp=0.001, is_pblocked=False, sub_debug=False, ncore=3, seed=None,
                           precision=_lib.MIXED_BATCHED, sweep_groups=0):
     """R/sampling.R:446-502: ncore independent single-subject runs."""
     model, data = dmi
@@ -152,7 +161,8 @@ def StartSampling_subject(dmi, priors, samples_list=None, nmc=500, nchain=None, 
     if samples_list is None:
         samples_list = [initialise_theta(nchain, priors, dmi, s, max_init_attempts) for s in seeds]
     kw = dict(sub_migration_prob=sub_migration_prob, gamma_precursor=gamma_precursor, rp=rp,
-              is_pblocked=is_pblocked, precision=precision, sweep_groups=sweep_groups)
+              is_pblocked=is_pblocked, precision=precision, sweep_groups=sweep_groups,
+              sub_debug=sub_debug)
     cfg = _config(nmc, nchain, thin, kw)
     jobs = [JobSpec(_lib.KIND_SUBJECT, sd, priors["p_prior"], start=st, subject=data,
                     nparameter_de=priors["nparameter"]) for sd, st in zip(seeds, samples_list)]
@@ -172,7 +182,8 @@ def RestartSampling_subject(samples_list, nmc=500, thin=1, **kw):
 def StartSampling(dmis, priors, samples_list=None, nmc=500, nchain=None, thin=1,
                   max_init_attempts=1000, pop_migration_prob=0.0, sub_migration_prob=0.0,
                   gamma_precursor=2.38, rp=0.001, is_hblocked=False, is_pblocked=False,
-                  ncore=3, seed=None, precision=_lib.MIXED_BATCHED, sweep_groups=0):
+                  pop_debug=False, sub_debug=False, ncore=3, seed=None,
+                  precision=_lib.MIXED_BATCHED, sweep_groups=0):
     """R/sampling.R:247-304: ncore independent hierarchical runs; each result
     is {'phi': Posterior, 'subject_theta': [Posterior]}."""
     model = dmis[0][0]
@@ -183,7 +194,8 @@ def StartSampling(dmis, priors, samples_list=None, nmc=500, nchain=None, thin=1,
                         for s in seeds]
     kw = dict(pop_migration_prob=pop_migration_prob, sub_migration_prob=sub_migration_prob,
               gamma_precursor=gamma_precursor, rp=rp, is_hblocked=is_hblocked,
-              is_pblocked=is_pblocked, precision=precision, sweep_groups=sweep_groups)
+              is_pblocked=is_pblocked, precision=precision, sweep_groups=sweep_groups,
+              pop_debug=pop_debug, sub_debug=sub_debug)
     cfg = _config(nmc, nchain, thin, kw)
     jobs = [JobSpec(_lib.KIND_HIER, sd, priors["p_prior"], h_prior=priors["h_prior"],
                     phi_start=st[0], subjects=[d[1] for d in dmis], subject_starts=st[1],
@@ -203,7 +215,7 @@ def RestartSampling(samples_list, nmc=500, thin=1, **kw):
 
 def StartSampling_hyper(hyper_dmi, dmis, priors, samples_list=None, nmc=500, nchain=None, thin=1,
                         max_init_attempts=1000, sub_migration_prob=0.0, gamma_precursor=2.38,
-                        rp=0.001, is_pblocked=False, ncore=3, seed=None,
+                        rp=0.001, is_pblocked=False, sub_debug=False, ncore=3, seed=None,
                         precision=_lib.MIXED_BATCHED, sweep_groups=0):
     """R/sampling.R:615-676: population parameters of fixed subject vectors
     hyper_dmi [S, P] (the hyper_dmi data). As in the reference, the start of
@@ -219,7 +231,8 @@ def StartSampling_hyper(hyper_dmi, dmis, priors, samples_list=None, nmc=500, nch
         samples_list = [initialise_phi(nchain, priors, dmis, sd % 1000000, max_init_attempts)[0]
                         for sd in seeds]
     kw = dict(sub_migration_prob=sub_migration_prob, gamma_precursor=gamma_precursor, rp=rp,
-              is_pblocked=is_pblocked, precision=precision, sweep_groups=sweep_groups)
+              is_pblocked=is_pblocked, precision=precision, sweep_groups=sweep_groups,
+              sub_debug=sub_debug)
     cfg = _config(nmc, nchain, thin, kw)
     hd = np.asarray(hyper_dmi, float)
     # the start's cached likelihood is initialise_phi's (against the subjects'
@@ -247,7 +260,7 @@ def _restart(prev, nmc, thin, kw):
     cfgd = dict(prev.configs)
     cfgd.update({k: v for k, v in kw.items() if k in cfgd or k in (
         "pop_migration_prob", "sub_migration_prob", "gamma_precursor", "rp", "is_hblocked",
-        "is_pblocked", "precision", "sweep_groups")})
+        "is_pblocked", "precision", "sweep_groups", "pop_debug", "sub_debug")})
     seeds = kw.get("seeds", cfgd["seeds"])
     cfgd["iter_offset"] = prev.iterations_done
     cfg = _config(nmc, cfgd["nchain"], thin, cfgd)

@@ -72,7 +72,29 @@ typedef struct {
                                also waits for its subject's previous iteration); never
                                changes results */
     uint64_t iter_offset;   /* Philox iteration offset (restart continues a stream) */
+    int32_t pop_debug;      /* de_input@pop_debug: record the population level's decisions */
+    int32_t sub_debug;      /* de_input@sub_debug: record the decisions of single-subject and
+                               hyper-only runs and of the first three subjects of hierarchical
+                               runs (src/de.cpp:85-106, :327) */
 } ggdmc_config;
+
+/* One Metropolis decision of a traced level (ggdmc_config pop_debug /
+ * sub_debug), what src/de.cpp prints with debug on (:85-106 update_theta,
+ * :441-463 population crossover, :533-558 population migration). */
+typedef struct {
+    int32_t job;         /* run index in the batch */
+    int32_t subject;     /* subject index within the run; -1: the population level */
+    uint32_t iteration;  /* 1 .. nmc*thin - 1 within this call or session */
+    int32_t sweep;       /* blocked parameter index, 0 for joint updates */
+    int32_t kind;        /* 0 crossover, 1 migration */
+    int32_t step;        /* crossover: the proposing chain; migration: the ring step */
+    int32_t chain;       /* proposing chain (crossover) / source chain (migration) */
+    int32_t target;      /* chain whose state the proposal would replace */
+    double tmp_lp, tmp_ll; /* the proposal's log prior and log likelihood */
+    double cur_post;     /* the target's cached log posterior it is compared with */
+    int32_t accepted;    /* 1 accepted, 0 rejected, -1 rejected (NaN ratio) */
+    int32_t reserved;
+} ggdmc_trace_record;
 
 /* Flattened LBA design: family f (0 A, 1 B, 2 mean_v, 3 sd_v, 4 st0, 5 t0) of
  * accumulator j in cell c is theta[par_src[(c*n_acc+j)*6+f]], or
@@ -191,6 +213,17 @@ void ggdmc_session_destroy(ggdmc_session *s);
 int ggdmc_session_decomposition(ggdmc_session *s, int32_t *chains_per_item,
                                 int32_t *trials_per_tile, int32_t *items_per_launch,
                                 int32_t *launch_mode);
+
+/* Decisions recorded during the last advance of a session (pop_debug /
+ * sub_debug), in the reference's print order: by run, iteration, level
+ * (population first, then subjects in order), sweep and step. *records stays
+ * valid until the next advance or destroy. At most 2^20 records are kept per
+ * advance; *dropped counts the rest. */
+int ggdmc_session_trace(ggdmc_session *s, const ggdmc_trace_record **records, int64_t *n,
+                        int64_t *dropped);
+/* The same for the last ggdmc_run_subject / ggdmc_run_hyper / ggdmc_run /
+ * ggdmc_run_batch call on the calling thread (valid until its next call). */
+int ggdmc_last_trace(const ggdmc_trace_record **records, int64_t *n, int64_t *dropped);
 
 const char *ggdmc_last_error(void);
 /* Number of visible gfx950 devices (0 without a GPU; never fails). */

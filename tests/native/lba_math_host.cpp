@@ -2,6 +2,7 @@
 // The transcendental policy uses libm in place of the FP32 hardware
 // instructions, so the formulation can be checked against the CPU oracle
 // without a GPU.
+#include "../../ggdmc_amd/csrc/fp64_math.h"
 #include "../../ggdmc_amd/csrc/lba_math.h"
 
 #include <cmath>
@@ -19,11 +20,11 @@ struct HostOps {
     static float exp2(float x) { return std::exp2(x); }
     static float log2(float x) { return std::log2(x); }
     static float rcp(float x) { return 1.0f / x; }
-    static double erfc(double x) { return std::erfc(x); }
-    static double pnorm(double z) {
-        return z > 0.0 ? 1.0 - 0.5 * std::erfc(z * ggdmc::kSqrt1_2) : 0.5 * std::erfc(-z * ggdmc::kSqrt1_2);
-    }
-    static double dnorm(double z) { return ggdmc::kInvSqrt2Pi * std::exp(-0.5 * z * z); }
+    /* the kernels' own FP64 functions, as on the device */
+    static double qtail(double a) { return ggdmc::qtail_acc(a); }
+    static double pnorm(double z) { return ggdmc::pnorm_acc(z); }
+    static double dnorm(double z) { return ggdmc::kInvSqrt2Pi * ggdmc::exp_acc(-0.5 * z * z); }
+    static double log(double x) { return ggdmc::log_acc(x); }
 };
 } // namespace
 

@@ -71,3 +71,88 @@ def test_errors_come_back_as_exceptions(gpu):
     with pytest.raises(gpu.GgdmcError):
         gpu.run_subject(gpu.make_config(nmc=5, nchain=18), w.model, wrong, job.subject,
                         job.start, seed=1)
+
+
+def _trace_run(gpu, cfg, model, jobs):
+    s = gpu.Session(cfg, model, jobs)
+    try:
+        s.advance(s.total_iters)
+        rec, dropped = s.trace()
+        return rec, dropped, s.download()
+    finally:
+        s.close()
+
+
+def _check_level(rec, th, lp, ll, nchain, nmc, population=False):
+    # every decision of every iteration, in the reference's order: a
+    # crossover sweep has one record per chain in chain order; a migration
+    # ring one per step, each target written once. An accepted decision is a
+    # changed state carrying the proposal's lp / ll; the comparison is always
+    # with the target's posterior before the iteration
+    assert np.all(np.diff(rec["iteration"].astype(int)) >= 0)
+    for it in range(1, nmc):
+        r = rec[rec["iteration"] == it]
+        assert len(r) > 0
+        if r["kind"][0] == 0:
+            assert list(r["step"]) == list(range(nchain)) and np.all(r["kind"] == 0)
+        else:
+            assert list(r["step"]) == list(range(len(r))) and np.all(r["kind"] == 1)
+            assert len(set(r["target"])) == len(r)
+        for x in r:
+            t = x["target"]
+            changed = bool(np.any(th[:, t, it] != th[:, t, it - 1]))
+            assert changed == (x["accepted"] == 1)
+            if population:
+                # the population level compares with its refreshed likelihood
+                # (src/de.cpp:397, :494-500), which later ring steps refresh again
+                if changed:
+                    assert x["tmp_lp"] == lp[t, it]
+                continue
+            assert x["cur_post"] == lp[t, it - 1] + ll[t, it - 1]
+            if changed:
+                assert x["tmp_lp"] == lp[t, it] and x["tmp_ll"] == ll[t, it]
+
+
+def test_debug_trace_follows_the_trajectory(gpu):
+    fix = fx.load("lba_data6")
+    n, nmc = 12, 8
+    start = fx.near_start(fix, n, seed=7)
+    job = gpu.JobSpec(gpu.KIND_SUBJECT, 9032, fix.sub_prior, start=start, subject=fix.sub_data)
+    for prec, groups in ((gpu.FP64_PLAIN, n), (gpu.FP64_BATCHED, 2), (gpu.MIXED_BATCHED, 2)):
+        cfg = gpu.make_config(nmc=nmc, nchain=n, precision=prec, sweep_groups=groups,
+                              sub_migration_prob=0.3, sub_debug=True)
+        rec, dropped, res = _trace_run(gpu, cfg, fix.model, [job])
+        post = res[0]["subjects"][0]
+        assert dropped == 0 and np.all(rec["subject"] == 0)
+        _check_level(rec, post.theta, post.summed_log_prior, post.log_likelihoods, n, nmc)
+        assert "accepted" in gpu.runs.format_trace(rec)
+    # no debug flag, no records
+    rec, _, _ = _trace_run(gpu, gpu.make_config(nmc=3, nchain=n), fix.model, [job])
+    assert len(rec) == 0
+
+
+def test_debug_trace_of_a_hierarchical_run(gpu):
+    fix = fx.load("lba_data6")
+    n, nmc, S = 10, 6, 4
+    phi, subs = fx.hier_start(fix, S, n, seed=3)
+    job = gpu.JobSpec(gpu.KIND_HIER, 77, fix.pop_prior, h_prior=fix.hyper_prior, phi_start=phi,
+                      subjects=[fix.pop_data(s) for s in range(S)], subject_starts=subs,
+                      nparameter_de=2 * fix.P)
+    for prec, groups in ((gpu.FP64_PLAIN, n), (gpu.MIXED_BATCHED, 2)):
+        cfg = gpu.make_config(nmc=nmc, nchain=n, precision=prec, sweep_groups=groups,
+                              pop_migration_prob=0.3, sub_migration_prob=0.3, pop_debug=True,
+                              sub_debug=True)
+        rec, dropped, res = _trace_run(gpu, cfg, fix.model, [job])
+        assert dropped == 0
+        # the population level and the first three subjects only (src/de.cpp:327)
+        assert set(rec["subject"]) == {-1, 0, 1, 2}
+        p = res[0]["phi"]
+        _check_level(rec[rec["subject"] == -1], p.theta, p.summed_log_prior, p.log_likelihoods, n, nmc,
+                     population=True)
+        for s in range(3):
+            q = res[0]["subjects"][s]
+            r = rec[rec["subject"] == s]
+            for it in range(1, nmc):
+                for x in r[r["iteration"] == it]:
+                    t = x["target"]
+                    assert bool(np.any(q.theta[:, t, it] != q.theta[:, t, it - 1])) == (x["accepted"] == 1)

@@ -184,3 +184,77 @@ def test_scale_workload_full_size(gpu):
     assert worst <= 1e-6
     assert moved > 0
     assert np.all(np.isfinite(res["phi"].theta))
+
+
+def test_batching_workload_full_size(gpu):
+    # the batching workload at its configured size: 256 subjects with ragged
+    # trial counts U{96..2048} in one hierarchical run plus 64 independent
+    # single-subject fits, 39 chains, one session (the reference runs them as
+    # separate processes, R/sampling.R:13-121). Every fit and a spread of
+    # subjects (the shortest and longest included): stored log-likelihoods
+    # within the per-proposal tolerance of an FP64 recomputation, every state
+    # finite, chains moving
+    w = workloads.make("c4")
+    hier = w.jobs[0]
+    ntr = np.array([len(d.rt) for d in hier.subjects])
+    assert len(hier.subjects) == 256 and len(w.jobs) == 65
+    assert ntr.min() < 150 and ntr.max() > 1950
+    cfg = w.config(nmc=9, thin=4, precision=gpu.MIXED_BATCHED)     # 35 iterations
+    s = gpu.Session(cfg, w.model, w.jobs)
+    s.advance(s.total_iters)
+    res = s.download()
+    print("c4 decomposition", s.decomposition())
+    s.close()
+    check = sorted(set(list(range(0, 256, 17)) + [int(ntr.argmin()), int(ntr.argmax())]))
+    pairs = [(hier.subjects[i], res[0]["subjects"][i], hier.subject_starts[i][0]) for i in check]
+    pairs += [(j.subject, r["subjects"][0], j.start[0]) for j, r in zip(w.jobs[1:], res[1:])]
+    worst = 0.0
+    moved = 0
+    for data, post, th0 in pairs:
+        assert np.all(np.isfinite(post.theta)) and np.all(np.isfinite(post.log_likelihoods))
+        ref = gpu.loglike(w.model, data, post.theta[:, :, -1].T, gpu.FP64_PLAIN)
+        worst = max(worst, float(np.max(np.abs(post.log_likelihoods[:, -1] / ref - 1))))
+        moved += int(np.any(post.theta[:, :, -1] != th0))
+    print("c4: %d states checked, stored-LL max rel %.2e, %d of %d runs moved"
+          % (39 * len(pairs), worst, moved, len(pairs)))
+    assert worst <= 1e-6
+    assert moved == len(pairs)
+    assert np.all(np.isfinite(res[0]["phi"].theta))
+
+
+def test_ragged_batch_follows_oracle(gpu):
+    # FP64 arithmetic, two chain groups, trial tiles of 256: a hierarchical run
+    # whose subjects have 96 and 2048 trials (and two in between) batched with
+    # two single-subject fits follows the oracle draw for draw, every run
+    w = workloads.make("c4", scale=1 / 32, ll_fn=_oracle_ll)     # for its model and two fits
+    from ggdmc_amd.workloads import BV_MEAN, BV_SCALE, _hier_job
+    hier, _ = _hier_job(w.model, BV_MEAN, BV_SCALE, [96, 2048, 700, 1500], 12, 9032, 77, _oracle_ll)
+    fits = [gpu.JobSpec(j.kind, j.seed, j.p_prior, start=(j.start[0][:, :12], j.start[1][:12],
+                                                             j.start[2][:12]),
+                        subject=j.subject, nparameter_de=13) for j in w.jobs[1:3]]
+    jobs = [hier] + fits
+    cfg = gpu.make_config(nmc=5, nchain=12, precision=gpu.FP64_BATCHED, sweep_groups=2,
+                          trials_per_tile=256, sub_migration_prob=0.2, pop_migration_prob=0.2)
+    res = gpu.run_batch(cfg, w.model, jobs)
+    om = O.Model(w.model.winner, w.model.par_src, w.model.par_const, 13)
+    ocfg = lambda npde, seed: O.config(nmc=5, nchain=12, nparameter_de=npde, sweep_groups=2,  # noqa: E731
+                                       sub_migration_prob=0.2, pop_migration_prob=0.2, seed=seed)
+    rt = np.concatenate([d.rt for d in hier.subjects])
+    so = np.concatenate([[0], np.cumsum([len(d.rt) for d in hier.subjects])])
+    co = np.stack([d.cell_off for d in hier.subjects])
+    o = O.run_hier(ocfg(26, hier.seed), om, fx.oprior(hier.p_prior), fx.oprior(hier.h_prior), rt, so,
+                   co, *hier.phi_start, np.stack([s[0] for s in hier.subject_starts]),
+                   np.stack([s[1] for s in hier.subject_starts]),
+                   np.stack([s[2] for s in hier.subject_starts]))
+    tol = dict(rtol=1e-9, atol=1e-12)
+    assert np.allclose(res[0]["phi"].theta, o[0], **tol)
+    assert np.allclose(res[0]["phi"].log_likelihoods, o[2], **tol)
+    for s in range(4):
+        assert np.allclose(res[0]["subjects"][s].theta, o[3][s], **tol)
+        assert np.allclose(res[0]["subjects"][s].log_likelihoods, o[5][s], **tol)
+    for j, r in zip(fits, res[1:]):
+        ot, olp, oll = O.run_subject(ocfg(13, j.seed), om, fx.oprior(j.p_prior), j.subject.rt,
+                                     j.subject.cell_off, *j.start)
+        assert np.allclose(r["subjects"][0].theta, ot, **tol)
+        assert np.allclose(r["subjects"][0].log_likelihoods, oll, **tol)
+    assert np.any(res[0]["subjects"][1].theta[:, :, -1] != hier.subject_starts[1][0])

@@ -343,3 +343,19 @@ def test_exhausted_initialisation_raises(monkeypatch):
     with pytest.raises(RuntimeError, match="Failed to find valid phi"):
         sampling.StartSampling_hyper(np.ones((2, P)), dmis, priors, nchain=3, ncore=1, seed=2,
                                      max_init_attempts=5)
+
+
+def test_shim_resolves_the_start_column_from_the_samples_object():
+    # a fresh samples object (set_up_new_samples + initialise_theta: column 1
+    # filled) starts from column 1; a finished posterior from its last column;
+    # a partly filled one from its last filled column
+    import shim_lib as SH
+    L = SH.load()
+    th = np.arange(2 * 3, dtype=float).reshape(2, 3)
+    for filled, want in ((1, 0), (5, 4), (3, 2)):
+        x = SH.r_samples(np.repeat(th[:, :, None], filled, 2), np.zeros((3, filled)),
+                         np.zeros((3, filled)), nmc_in=5, filled=filled)[0]
+        assert L.shim_start_column(_lib.ptr(x), 2, 3, 5) == want
+    x = SH.r_samples(th, np.zeros(3), np.zeros(3), nmc_in=4)[0]
+    x[1, 2, 3] = 1.0          # one finite entry does not make a column
+    assert L.shim_start_column(_lib.ptr(x), 2, 3, 4) == 0
