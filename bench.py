@@ -209,7 +209,9 @@ def main():
           % (w.name, w.description, w.n_subjects, w.n_trials,
              "all groups" if kci < 0 else kci, tpt or "all", nitems,
              {1: "a subject launch per chain group", 2: "one subject launch for every chain group",
-              3: "one subject launch for every chain group and 32 iterations"}[lmode],
+              3: "one subject launch for every chain group and 32 iterations",
+              4: "one subject launch for every chain group and 32 iterations, an item "
+                 "holding its chain range of every group"}[lmode],
              time.time() - t_build),
           file=sys.stderr, flush=True)
 

@@ -139,6 +139,8 @@ struct SweepArgs {
                              grid is iters segments of seg_wgs workgroups, segment s running
                              iteration *iter_base + iter_off + s */
     int32_t seg_wgs;      /* phi_wgs + n_items rounded up to a multiple of 8 (iters > 1) */
+    int32_t span;         /* chained: an item holds its chain range of every group, in group
+                             order (items carry group 0); else one group per item */
 };
 
 } // namespace ggdmc

@@ -112,6 +112,21 @@ struct Pool {
     }
 };
 
+int device_cus() {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    return cus;
+}
+
+/* Items spanning every group wait, in their later groups, on items of their
+ * subject with higher workgroup ids as well as lower ones. The items of a
+ * subject lie within kXcd * ranges consecutive workgroups (XCD-aware order);
+ * while that window fits one workgroup per CU every item it waits on is
+ * resident or done, whatever else occupies the chip. */
+constexpr int kXcd = 8;
+bool span_fits(int ranges) { return kXcd * ranges <= device_cus(); }
+
 int device_lds_limit() {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 65536;
@@ -146,6 +161,7 @@ struct ggdmc_session {
     bool overlap = false;
     bool chained = false;         /* every chain group in one subject launch */
     int ipl = 1;                  /* iterations per subject launch (chained only) */
+    bool span = false;            /* chained: an item holds its chain range of every group */
     bool tiled = false;           /* some subject's trials split over several items */
     int rmax = 1;                 /* chain ranges per group (item_ready stride) */
     int ovl_stats_lds = 0;
@@ -179,7 +195,8 @@ struct ggdmc_session {
     }
 
     void build(const ggdmc_config *c, const ggdmc_model *model, const ggdmc_job *js, int nj);
-    void set_decomp(bool fused_new, int k_new, int tpt_new, bool chained_new = false, int ipl_new = 1);
+    void set_decomp(bool fused_new, int k_new, int tpt_new, bool chained_new = false, int ipl_new = 1,
+                    bool span_new = false);
     void default_decomp();
     void enqueue_iteration(uint32_t off, std::vector<hipEvent_t> *ev = nullptr, int count = 1);
     void flush_store();
@@ -578,19 +595,20 @@ void ggdmc_session::build(const ggdmc_config *c, const ggdmc_model *model, const
  * fill the chip: chains per item so that subjects x chain ranges >= 1024. */
 void ggdmc_session::default_decomp() {
     const int ngmax = (d.nchain + d.G - 1) / d.G;
-    require(cfg.launch_mode >= 0 && cfg.launch_mode <= 3, "launch_mode must be 0, 1, 2 or 3");
+    require(cfg.launch_mode >= 0 && cfg.launch_mode <= 4, "launch_mode must be 0, 1, 2, 3 or 4");
     const bool one = cfg.launch_mode >= 2;
-    const int ipl_cfg = cfg.launch_mode == 3 ? kItersPerLaunch : 1;
+    const int ipl_cfg = cfg.launch_mode >= 3 ? kItersPerLaunch : 1;
+    const bool span_cfg = cfg.launch_mode == 4;
     if (cfg.chains_per_item < 0) {
         require(cfg.trials_per_tile == 0, "one item per subject needs trials_per_tile = 0");
         require(!one, "one item per subject already covers every group in one launch");
         set_decomp(true, 0, 0);
     } else if (cfg.chains_per_item > 0 || cfg.trials_per_tile > 0) {
-        set_decomp(false, cfg.chains_per_item, cfg.trials_per_tile, one, ipl_cfg);
+        set_decomp(false, cfg.chains_per_item, cfg.trials_per_tile, one, ipl_cfg, span_cfg);
     } else {
         int k = ngmax;
         while (k > 1 && (int64_t)d.S * ((ngmax + k - 1) / k) < 1024) k = (k + 1) / 2;
-        set_decomp(false, k, 0, one, ipl_cfg);
+        set_decomp(false, k, 0, one, ipl_cfg, span_cfg);
     }
 }
 
@@ -598,17 +616,21 @@ void ggdmc_session::default_decomp() {
  * subject's chain ranges of k_new chains x trial tiles of tpt_new trials
  * (rounded up to a multiple of kChunk; 0 = whole subject); or, fused, one
  * item per subject that sweeps every group in one launch. Sizes the LDS. */
-void ggdmc_session::set_decomp(bool fused_new, int k_new, int tpt_new, bool chained_new, int ipl_new) {
+void ggdmc_session::set_decomp(bool fused_new, int k_new, int tpt_new, bool chained_new, int ipl_new,
+                               bool span_new) {
     if (tpt_new > 0) tpt_new = ((tpt_new + kChunk - 1) / kChunk) * kChunk;
     require(!fused_new || tpt_new == 0, "one item per subject needs whole subjects");
     require(!(fused_new && chained_new), "one item per subject already covers every group");
     require(!chained_new || nsweep_sub == 1, "every group in one launch needs unblocked subject sweeps");
     require(ipl_new == 1 || chained_new, "several iterations per launch need every group in one launch");
+    require(!span_new || (chained_new && tpt_new == 0),
+            "items spanning every group need every group in one launch and whole subjects");
     const int n = d.nchain, G = d.G;
     const int ngmax = (n + G - 1) / G;
     fused = fused_new;
     chained = chained_new;
     ipl = ipl_new;
+    span = span_new;
     tiled = false;
     tpt = tpt_new;
     d.tpt = tpt;
@@ -616,14 +638,13 @@ void ggdmc_session::set_decomp(bool fused_new, int k_new, int tpt_new, bool chai
     lists.clear();
     list_n.clear();
     int64_t max_chunks_item = 1;
-    const int nl = fused ? 1 : G;
+    const int nl = (fused || span) ? 1 : G;
     std::vector<int4> all_items;   /* every group's items, in group order (chained) */
     rmax = 1;
     /* XCD-aware order: workgroup w runs on XCD w % 8, so the items of subject
      * x go to workgroups w = x (mod 8) where possible, and share that XCD's
      * L2 for the subject's trials and state. The population workgroups come
      * first in a launch when the population step runs inside it. */
-    constexpr int kXcd = 8;
     const bool phi_in = d.n_phi > 0 && d.S > 0 && !fused && nsweep_sub == 1;
     int64_t wg_base = phi_in && chained ? (int64_t)d.n_phi * n : 0;
     auto xcd_order = [&](std::vector<int4> &v, int64_t base) {
@@ -644,7 +665,7 @@ void ggdmc_session::set_decomp(bool fused_new, int k_new, int tpt_new, bool chai
         v.swap(out);
     };
     for (int g = 0; g < nl; ++g) {
-        const int ng = fused ? ngmax : (((g + 1) * n) / G - (g * n) / G);
+        const int ng = (fused || span) ? ngmax : (((g + 1) * n) / G - (g * n) / G);
         const int nch = fused ? 1 : (ng + kchain - 1) / kchain;
         std::vector<int4> items;
         for (int x = 0; x < d.S; ++x) {
@@ -663,6 +684,7 @@ void ggdmc_session::set_decomp(bool fused_new, int k_new, int tpt_new, bool chai
                                                 nt == 1 ? (ntr + kChunk - 1) / kChunk : tpt / kChunk);
         }
         rmax = std::max(rmax, nch);
+        require(!span || span_fits(nch), "too many chain ranges per subject for items spanning every group");
         if (chained) {
             xcd_order(items, wg_base);
             wg_base += (int64_t)items.size();
@@ -750,7 +772,7 @@ void ggdmc_session::enqueue_iteration(uint32_t off, std::vector<hipEvent_t> *ev,
             const int seg_wgs = multi > 1 ? (pw + list_n[g] + 7) / 8 * 8 : pw + list_n[g];
             SweepArgs a{off, sw, (fused || chained) ? 0 : g, (fused || chained) ? d.G : g + 1, (int)store,
                         chain_batch, kmax, cmax, tstage, list_n[g], lists[g],
-                        pw, ovl_stats_lds, (int)chained, rmax, (int)tiled, multi, seg_wgs};
+                        pw, ovl_stats_lds, (int)chained, rmax, (int)tiled, multi, seg_wgs, (int)span};
             hipEvent_t e0 = nullptr, e1 = nullptr;
             if (ev) {
                 HIPCHECK(hipEventCreate(&e0));
@@ -842,9 +864,10 @@ void ggdmc_session::autotune() {
         int k, tpt;
         bool chained;
         int ipl;
+        bool span;
     };
     const int ngmax = (d.nchain + d.G - 1) / d.G;
-    std::vector<Cand> cands{{true, 0, 0, false, 1}};
+    std::vector<Cand> cands{{true, 0, 0, false, 1, false}};
     std::vector<int> ks;
     for (int k : {1, 2, 4, 8, ngmax})
         if (k <= ngmax && std::find(ks.begin(), ks.end(), k) == ks.end()) ks.push_back(k);
@@ -855,11 +878,15 @@ void ggdmc_session::autotune() {
                 if (t == 0 || t < max_ntr) {
                     const int64_t nt = t == 0 ? 1 : (max_ntr + t - 1) / t;
                     if ((int64_t)d.S * ((ngmax + k - 1) / k) * nt <= ((int64_t)1 << 22)) {
-                        cands.push_back({false, k, t, one == 1, 1});
+                        cands.push_back({false, k, t, one == 1, 1, false});
                         /* several iterations per launch: whole subjects, the
-                         * population step inside the launch */
-                        if (one == 1 && t == 0 && cfg.launch_mode != 2)
-                            cands.push_back({false, k, t, true, kItersPerLaunch});
+                         * population step inside the launch; items of one
+                         * group, or spanning every group */
+                        if (one == 1 && t == 0 && cfg.launch_mode != 2) {
+                            cands.push_back({false, k, t, true, kItersPerLaunch, false});
+                            if (span_fits((ngmax + k - 1) / k))
+                                cands.push_back({false, k, t, true, kItersPerLaunch, true});
+                        }
                     }
                 }
     const size_t nth = (size_t)d.S * d.nchain * d.P, nch = (size_t)d.S * d.nchain;
@@ -905,10 +932,10 @@ void ggdmc_session::autotune() {
     const uint32_t it0 = next_iter;
     float best = 1e30f;
     Cand pick = cands[0];
-    const int K = 8;   /* iterations per timed replay (one launch in launch mode 3) */
+    const int K = 16;  /* iterations per timed replay (one launch in launch modes 3 and 4) */
     for (const Cand &c : cands) {
         try {
-            set_decomp(c.fused, c.k, c.tpt, c.chained, c.ipl);
+            set_decomp(c.fused, c.k, c.tpt, c.chained, c.ipl, c.span);
         } catch (const Error &) {
             if (c.ipl == 1) throw;
             continue;   /* no population step inside the launch for this design */
@@ -930,7 +957,7 @@ void ggdmc_session::autotune() {
         }
     }
     next_iter = it0;
-    set_decomp(pick.fused, pick.k, pick.tpt, pick.chained, pick.ipl);
+    set_decomp(pick.fused, pick.k, pick.tpt, pick.chained, pick.ipl, pick.span);
 }
 
 static int fail(const std::exception &e) {
@@ -1063,7 +1090,7 @@ int ggdmc_session_decomposition(ggdmc_session *s, int32_t *chains_per_item,
                                 int32_t *trials_per_tile, int32_t *items_per_launch,
                                 int32_t *launch_mode) {
     if (!s) return invalid("session is null");
-    if (launch_mode) *launch_mode = s->ipl > 1 ? 3 : (s->chained ? 2 : 1);
+    if (launch_mode) *launch_mode = s->span ? 4 : (s->ipl > 1 ? 3 : (s->chained ? 2 : 1));
     if (chains_per_item) *chains_per_item = s->fused ? -1 : s->kchain;
     if (trials_per_tile) *trials_per_tile = s->tpt;
     if (items_per_launch)

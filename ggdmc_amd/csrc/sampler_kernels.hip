@@ -1734,7 +1734,7 @@ __global__ __launch_bounds__(kBlock, 3) void subject_sweep_kernel(Dev d, SweepAr
     const int n = d.nchain, P = d.P, G = d.G;
     /* the item's chain groups [g_begin, g_end) and chain range r */
     const int g_begin = a.chained ? item.w >> 16 : a.g_begin;
-    const int g_end = a.chained ? g_begin + 1 : a.g_end;
+    const int g_end = a.chained ? (a.span ? G : g_begin + 1) : a.g_end;
     const int rng = item.w & 0xFFFF;
     const SweepLds L = carve(lds_raw, n, P, a.kmax, d.n_cond * d.n_acc, a.chain_batch, a.cmax, a.tstage);
     /* the item's trials; staged in LDS when they fit */
@@ -1850,18 +1850,19 @@ __global__ __launch_bounds__(kBlock, 3) void subject_sweep_kernel(Dev d, SweepAr
     /* this item has read the subject's rows: a later group may change them */
     if (a.chained && t == 0 && threadIdx.x == 0)
         __hip_atomic_store(staged + (size_t)g_begin * a.rmax + rng, iter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (a.chained && !migrate && g_begin > 0) {
-        /* the previous group's items of this subject first; then the rows
-         * of the earlier groups' chains, which they changed */
-        const int glo = group_lo(g_begin - 1, n, G);
-        const int nr = (group_lo(g_begin, n, G) - glo + a.kmax - 1) / a.kmax;
+    /* group gq > 0 of a subject in one launch: the previous group's items of
+     * this subject first; then the rows of the earlier groups' chains, which
+     * they changed */
+    auto wait_prev_group = [&](int gq) {
+        const int glo = group_lo(gq - 1, n, G);
+        const int nr = (group_lo(gq, n, G) - glo + a.kmax - 1) / a.kmax;
         if (G == 2 && !a.tiles) {
             /* two groups, whole subjects: every group-0 item of the subject
              * has read its rows, and the items holding this item's partners
              * (all in group 0) are done */
             wait_items(d, staged, nr, iter);
             int lo1, hi1, i01;
-            const int k1 = group_k(g_begin, lo1, hi1, i01);
+            const int k1 = group_k(gq, lo1, hi1, i01);
             for (int e = threadIdx.x; e < 2 * k1; e += blockDim.x) {
                 const int m = (e & 1) ? L.m1[e >> 1] : L.m0[e >> 1];
                 const uint32_t *f = ready + (m - glo) / a.kmax;
@@ -1877,13 +1878,14 @@ __global__ __launch_bounds__(kBlock, 3) void subject_sweep_kernel(Dev d, SweepAr
             }
             __syncthreads();
         } else {
-            wait_items(d, ready + (size_t)(g_begin - 1) * a.rmax, nr, iter);
+            wait_items(d, ready + (size_t)(gq - 1) * a.rmax, nr, iter);
         }
         const double *src = d.theta + (size_t)x * n * P;
-        for (int e = threadIdx.x; e < group_lo(g_begin, n, G) * P; e += blockDim.x)
+        for (int e = threadIdx.x; e < group_lo(gq, n, G) * P; e += blockDim.x)
             L.th[e] = __hip_atomic_load(src + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
-    }
+    };
+    if (a.chained && !migrate && g_begin > 0) wait_prev_group(g_begin);
 
     if (__builtin_expect(migrate, 0)) {
         /* with the population step in these launches, a hierarchical
@@ -1917,6 +1919,8 @@ __global__ __launch_bounds__(kBlock, 3) void subject_sweep_kernel(Dev d, SweepAr
             if (g != g_begin) {
                 proposal_stage1(d, job, L, st, lo, hi, i0, k, para, late);
                 __syncthreads();
+                /* an item spanning every group: this group's partners */
+                if (a.chained) wait_prev_group(g);
             }
             proposal_stage2(d, job, L, i0, k, para, late);
             __syncthreads();

@@ -152,7 +152,8 @@ def make_config(nmc, nchain, thin=1, pop_migration_prob=0.0, sub_migration_prob=
                 sub_debug=False):
     """ggdmc_config. trials_per_tile / chains_per_item / launch_mode 0 let the
     library autotune the work split; launch_mode 1 = a subject launch per chain
-    group, 2 = one for every group, 3 = one for every group and 32 iterations.
+    group, 2 = one for every group, 3 = one for every group and 32 iterations,
+    4 = as 3 with an item holding its chain range of every group.
     None of them changes a result bit. pop_debug / sub_debug record the
     decisions of the population level / of the subjects (Session.trace)."""
     return _lib.Config(int(nmc), int(thin), int(nchain), float(pop_migration_prob),

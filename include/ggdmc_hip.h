@@ -69,7 +69,9 @@ typedef struct {
                                (else 1), 1 = one per chain group, 2 = one for every chain
                                group (an item waits for the previous group's items of its
                                subject), 3 = as 2 with 32 iterations per launch (an item
-                               also waits for its subject's previous iteration); never
+                               also waits for its subject's previous iteration), 4 = as 3
+                               with an item holding its chain range of every group (its
+                               later groups wait for the earlier groups' items); never
                                changes results */
     uint64_t iter_offset;   /* Philox iteration offset (restart continues a stream) */
     int32_t pop_debug;      /* de_input@pop_debug: record the population level's decisions */
@@ -207,7 +209,7 @@ void ggdmc_session_destroy(ggdmc_session *s);
 /* Work decomposition of a session's subject launches (after create, so after
  * autotuning): chains per item (-1: one item per subject over every group),
  * trials per item (0: whole subject), the items of the largest launch and the
- * launch mode (1, 2 or 3, as ggdmc_config.launch_mode). The decomposition never
+ * launch mode (1 to 4, as ggdmc_config.launch_mode). The decomposition never
  * changes results: log-likelihoods are summed in a fixed chunk order whatever
  * the split. */
 int ggdmc_session_decomposition(ggdmc_session *s, int32_t *chains_per_item,

@@ -177,10 +177,11 @@ def test_decomposition_never_changes_results(gpu, precision):
     ref = None
     # (chains per item, trials per item, launch mode: 1 = a launch per chain
     # group, 2 = every group in one launch, later groups waiting for earlier,
-    # 3 = as 2 with several iterations per launch)
+    # 3 = as 2 with several iterations per launch, 4 = as 3 with an item
+    # holding its chain range of every group)
     for k, tpt, lm in ((-1, 0, 1), (0, 0, 1), (1, 0, 1), (4, 128, 1), (2, 256, 1), (0, 384, 1),
                        (0, 0, 2), (1, 0, 2), (4, 128, 2), (2, 256, 2), (0, 0, 3), (1, 0, 3),
-                       (2, 0, 3), (4, 128, 3)):
+                       (2, 0, 3), (4, 128, 3), (0, 0, 4), (1, 0, 4), (2, 0, 4), (5, 0, 4)):
         cfg = gpu.make_config(nmc=6, nchain=n, pop_migration_prob=0.2, sub_migration_prob=0.3,
                               precision=prec, sweep_groups=2, chains_per_item=k,
                               trials_per_tile=tpt, launch_mode=lm)
@@ -281,7 +282,7 @@ def test_hier_with_an_empty_subject_follows_oracle(gpu):
         assert not np.array_equal(r["subject_theta"][1].theta[:, :, -1], subs[1][0])
 
 
-@pytest.mark.parametrize("launch_mode", [2, 3])
+@pytest.mark.parametrize("launch_mode", [2, 3, 4])
 def test_advance_in_pieces_equals_one_advance(gpu, launch_mode):
     # a session advanced in uneven pieces (a restart's pattern; with several
     # iterations per launch the pieces end mid-launch) gives the bits of one
@@ -312,6 +313,32 @@ def test_advance_in_pieces_equals_one_advance(gpu, launch_mode):
     assert not np.array_equal(a["subjects"][0].theta[:, :, -1], subs[0][0])
 
 
+def test_three_chain_groups_in_every_launch_mode(gpu):
+    # three chain groups: with every group in one launch a group waits for all
+    # of the previous group's items, also inside an item that holds its chain
+    # range of every group; the schedule changes no bit
+    fix = fx.load("lba_data6")
+    S, n = 4, 10
+    (phi, plp, pll), subs = fx.hier_start(fix, S, n, seed=37)
+    datas = [fix.pop_data(s) for s in range(S)]
+    ref = None
+    for k, lm in ((0, 1), (1, 2), (1, 3), (1, 4), (2, 4), (4, 4)):
+        cfg = gpu.make_config(nmc=8, nchain=n, pop_migration_prob=0.2, sub_migration_prob=0.2,
+                              precision=gpu.FP64_BATCHED, sweep_groups=3, chains_per_item=k,
+                              launch_mode=lm)
+        r = gpu.run(cfg, fix.model, fix.pop_prior, fix.hyper_prior, datas, (phi, plp, pll), subs,
+                    seed=43)
+        if ref is None:
+            ref = r
+            continue
+        assert np.array_equal(r["phi"].theta, ref["phi"].theta), (k, lm)
+        for a, b in zip(r["subject_theta"], ref["subject_theta"]):
+            assert np.array_equal(a.theta, b.theta), (k, lm)
+            assert np.array_equal(a.log_likelihoods, b.log_likelihoods), (k, lm)
+            assert np.array_equal(a.summed_log_prior, b.summed_log_prior), (k, lm)
+    assert not np.array_equal(ref["subject_theta"][0].theta[:, :, -1], subs[0][0])
+
+
 def test_hblocked_population_in_every_launch_mode(gpu):
     # blocked population sweeps: every iteration takes the whole-run population
     # step (one workgroup of the launch); with several iterations per launch it
@@ -322,7 +349,7 @@ def test_hblocked_population_in_every_launch_mode(gpu):
     (phi, plp, pll), subs = fx.hier_start(fix, S, n, seed=29)
     datas = [fix.pop_data(s) for s in range(S)]
     ref = None
-    for lm in (1, 2, 3):
+    for lm in (1, 2, 3, 4):
         cfg = gpu.make_config(nmc=6, nchain=n, pop_migration_prob=0.2, sub_migration_prob=0.2,
                               precision=gpu.FP64_BATCHED, sweep_groups=2, is_hblocked=True,
                               chains_per_item=2, launch_mode=lm)
@@ -365,7 +392,7 @@ def test_hyper_and_hierarchical_batch_in_every_launch_mode(gpu):
             gpu.JobSpec(gpu.KIND_HYPER, 43, fix.pop_prior, h_prior=fix.hyper_prior,
                         phi_start=(phi, hlp, hll), hyper_data=hdata, nparameter_de=2 * P)]
     ref = None
-    for lm in (1, 2, 3):
+    for lm in (1, 2, 3, 4):
         cfg = gpu.make_config(nmc=6, nchain=n, pop_migration_prob=0.2, sub_migration_prob=0.2,
                               precision=gpu.MIXED_BATCHED, sweep_groups=2, chains_per_item=2,
                               launch_mode=lm)

@@ -102,13 +102,14 @@ def test_hierarchical_posterior_matches_reference_order(gpu):
     assert np.all(np.abs(rl - 1) < 0.25) and np.all(np.abs(rs_ - 1) < 0.25)
 
 
-@pytest.mark.parametrize("launch_mode", [1, 2, 3])
+@pytest.mark.parametrize("launch_mode", [1, 2, 3, 4])
 def test_batched_job_equals_job_alone(gpu, launch_mode):
     # a run's trajectory does not depend on the other runs of its batch
     w = workloads.make("c4", scale=1 / 16, ll_fn=_oracle_ll)    # 16 ragged subjects + 4 fits
+    # (items spanning every group, launch mode 4, take whole subjects)
     cfg = gpu.make_config(nmc=6, nchain=w.nchain, precision=gpu.MIXED_BATCHED, sweep_groups=2,
-                          trials_per_tile=256, sub_migration_prob=0.2, pop_migration_prob=0.2,
-                          launch_mode=launch_mode)
+                          trials_per_tile=256 if launch_mode < 4 else 0, sub_migration_prob=0.2,
+                          pop_migration_prob=0.2, launch_mode=launch_mode)
     together = gpu.run_batch(cfg, w.model, w.jobs)
     for k in (0, len(w.jobs) - 1):
         alone = gpu.run_batch(cfg, w.model, [w.jobs[k]])[0]
@@ -125,7 +126,8 @@ def test_mixed_batch_schedules_agree(gpu):
     # same bits (the waits between groups only order the work)
     w = workloads.make("c4", scale=1 / 16, ll_fn=_oracle_ll)
     ref = None
-    for k, tpt, lm in ((0, 0, 1), (2, 0, 2), (0, 256, 2), (4, 128, 2), (2, 0, 3), (0, 256, 3)):
+    for k, tpt, lm in ((0, 0, 1), (2, 0, 2), (0, 256, 2), (4, 128, 2), (2, 0, 3), (0, 256, 3),
+                       (1, 0, 4), (4, 0, 4)):
         cfg = gpu.make_config(nmc=6, nchain=w.nchain, precision=gpu.MIXED_BATCHED, sweep_groups=2,
                               chains_per_item=k, trials_per_tile=tpt, launch_mode=lm,
                               sub_migration_prob=0.2, pop_migration_prob=0.2)
@@ -143,7 +145,7 @@ def test_mixed_batch_schedules_agree(gpu):
                 assert np.array_equal(a["phi"].log_likelihoods, b["phi"].log_likelihoods), (k, tpt, lm)
 
 
-@pytest.mark.parametrize("launch_mode", [1, 2, 3])
+@pytest.mark.parametrize("launch_mode", [1, 2, 3, 4])
 def test_batch_of_hierarchical_runs_equals_each_alone(gpu, launch_mode):
     # three hierarchical runs (same data, different Philox keys) in one batch:
     # each gives the bits it gives alone, phi and subjects, with migration
