@@ -943,12 +943,17 @@ void ggdmc_session::autotune() {
         next_iter = it0;
         capture(K);
         run_batched(K); /* warm-up */
-        HIPCHECK(hipEventRecord(ev0, stream));
-        run_batched(2 * K);
-        HIPCHECK(hipEventRecord(ev1, stream));
-        HIPCHECK(hipEventSynchronize(ev1));
-        float ms = 0.f;
-        HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
+        /* the faster of two timed replays */
+        float ms = 1e30f;
+        for (int rep = 0; rep < 2; ++rep) {
+            HIPCHECK(hipEventRecord(ev0, stream));
+            run_batched(K);
+            HIPCHECK(hipEventRecord(ev1, stream));
+            HIPCHECK(hipEventSynchronize(ev1));
+            float m = 0.f;
+            HIPCHECK(hipEventElapsedTime(&m, ev0, ev1));
+            ms = std::min(ms, m);
+        }
         drop_graph();
         restore();
         if (ms < best) {
