@@ -47,16 +47,19 @@ W_ALG = {2: 130.0, 4: 255.0}
 # everything included (precision work, indexing, staging, sums, proposals,
 # the population workgroups, and the polling of items that wait for their
 # subject's previous iteration): rocprofv3 SQ_INSTS_VALU x 64 / evaluations of
-# a launch (profiles/), at the decomposition the autotuner picks there (c3: 2
-# chains x all trials per item, c5: a whole chain group per item; every chain
-# group and 32 iterations in one launch).
-# Keyed by (workload, chains per item, trials per item, launch mode): a figure
-# is reported only for the decomposition it was counted on.
-W_ISSUED = {("c3", 2, 0, 3): 751.0, ("c5", 17, 0, 3): 838.5}
-# HBM bytes per iteration of the subject sweep kernel at that decomposition:
-# rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors of a 256 MiB copy
-# kernel; profiles/) per launch, divided by the 32 iterations of a launch.
-TRAFFIC = {("c3", 2, 0, 3): 1.131e6, ("c5", 17, 0, 3): 2.124e7}
+# a launch of 32 iterations (profiles/subject_sweep_kernel_<workload>.md).
+# Keyed by (workload, chains per item, trials per item, launch mode), for the
+# decompositions the autotuner picks on these workloads: a figure is reported
+# only for the decomposition it was counted on.
+W_ISSUED = {("c3", 1, 0, 4): 787.6, ("c3", 1, 0, 3): 884.6, ("c3", 2, 0, 4): 663.9,
+            ("c3", 2, 0, 3): 710.4, ("c5", 17, 0, 3): 862.2, ("c5", 17, 0, 4): 862.9,
+            ("c4", 4, 0, 3): 589.5, ("c4", 4, 0, 4): 573.6}
+# HBM bytes per iteration of the subject sweep kernel at those decompositions:
+# rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB (factors of a 256 MiB copy
+# kernel, same files) per launch, divided by the 32 iterations of a launch.
+TRAFFIC = {("c3", 1, 0, 4): 1.572e6, ("c3", 1, 0, 3): 1.301e6, ("c3", 2, 0, 4): 1.311e6,
+           ("c3", 2, 0, 3): 1.282e6, ("c5", 17, 0, 3): 2.167e7, ("c5", 17, 0, 4): 2.206e7,
+           ("c4", 4, 0, 3): 5.486e6, ("c4", 4, 0, 4): 5.524e6}
 # Iterations a bench session holds at least, whatever --steps asks: the
 # library autotunes the work decomposition of sessions of 64 or more
 # iterations, so the measured configuration never depends on the step count.

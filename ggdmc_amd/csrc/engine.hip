@@ -173,6 +173,10 @@ struct ggdmc_session {
     uint32_t *iter_base = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_iters = 0;
+    /* a graph of the remainder of the last advance (fewer iterations than
+     * graph_iters), captured before that advance's timed region */
+    hipGraphExec_t rest_graph = nullptr;
+    int rest_iters = 0;
     float last_ms = 0.f;
     double *hyper_rows = nullptr;
     int64_t *hyper_off = nullptr;
@@ -186,6 +190,7 @@ struct ggdmc_session {
 
     ~ggdmc_session() {
         if (graph) (void)hipGraphExecDestroy(graph);
+        if (rest_graph) (void)hipGraphExecDestroy(rest_graph);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) {
@@ -204,6 +209,8 @@ struct ggdmc_session {
     void run_plain(uint32_t n_iter);
     void autotune();
     void capture(int K);
+    hipGraphExec_t make_graph(int K);
+    void prepare_rest(uint32_t n_iter);
     /* a subject workgroup gave up waiting for the population step */
     void check_fault() {
         uint32_t f = 0;
@@ -235,6 +242,9 @@ struct ggdmc_session {
         if (graph) (void)hipGraphExecDestroy(graph);
         graph = nullptr;
         graph_iters = 0;
+        if (rest_graph) (void)hipGraphExecDestroy(rest_graph);
+        rest_graph = nullptr;
+        rest_iters = 0;
     }
 };
 
@@ -795,9 +805,9 @@ void ggdmc_session::flush_store() {
         HIPCHECK(ggdmc_internal_launch_store(&d, it, 1, stream));
 }
 
-void ggdmc_session::capture(int K) {
-    drop_graph();
-    if (K < 2) return;
+/* The launches of K iterations (iteration base from the device word) as an
+ * instantiated graph. */
+hipGraphExec_t ggdmc_session::make_graph(int K) {
     hipGraph_t g = nullptr;
     HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     try {
@@ -809,10 +819,33 @@ void ggdmc_session::capture(int K) {
         throw;
     }
     HIPCHECK(hipStreamEndCapture(stream, &g));
-    const hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+    hipGraphExec_t x = nullptr;
+    const hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     HIPCHECK(e);
+    return x;
+}
+
+void ggdmc_session::capture(int K) {
+    drop_graph();
+    if (K < 2) return;
+    graph = make_graph(K);
     graph_iters = K;
+}
+
+/* An advance of n_iter iterations replays the K-iteration graph while it
+ * can; the rest (fewer than K iterations, as in a short run or a restart's
+ * pieces) gets a graph of its own, captured here, before the advance's timed
+ * region, and kept for the next advance of the same remainder. */
+void ggdmc_session::prepare_rest(uint32_t n_iter) {
+    if (!graph) return;
+    const int r = (int)(n_iter % (uint32_t)graph_iters);
+    if (r < 2 || r == rest_iters) return;
+    if (rest_graph) (void)hipGraphExecDestroy(rest_graph);
+    rest_graph = nullptr;
+    rest_iters = 0;
+    rest_graph = make_graph(r);
+    rest_iters = r;
 }
 
 void ggdmc_session::run_batched(uint32_t n_iter) {
@@ -824,6 +857,12 @@ void ggdmc_session::run_batched(uint32_t n_iter) {
             HIPCHECK(hipGraphLaunch(graph, stream));
             next_iter += graph_iters;
             n_iter -= graph_iters;
+        } else if (rest_graph && n_iter == (uint32_t)rest_iters) {
+            set_base_kernel<<<1, 1, 0, stream>>>(iter_base, next_iter);
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipGraphLaunch(rest_graph, stream));
+            next_iter += rest_iters;
+            n_iter = 0;
         } else {
             set_base_kernel<<<1, 1, 0, stream>>>(iter_base, next_iter);
             HIPCHECK(hipGetLastError());
@@ -932,7 +971,9 @@ void ggdmc_session::autotune() {
     const uint32_t it0 = next_iter;
     float best = 1e30f;
     Cand pick = cands[0];
-    const int K = 16;  /* iterations per timed replay (one launch in launch modes 3 and 4) */
+    /* iterations per timed replay: one launch of launch modes 3 and 4 when
+     * the run has room for three (warm-up and two timed) */
+    const int K = last_iter >= 3 * (uint32_t)kItersPerLaunch ? kItersPerLaunch : 16;
     for (const Cand &c : cands) {
         try {
             set_decomp(c.fused, c.k, c.tpt, c.chained, c.ipl, c.span);
@@ -1071,6 +1112,7 @@ int ggdmc_session_advance(ggdmc_session *s, int32_t n_iter) {
         const uint32_t left = s->next_iter <= s->last_iter ? s->last_iter - s->next_iter + 1 : 0;
         const uint32_t n = std::min<uint32_t>((uint32_t)n_iter, left);
         if (s->d.trace) HIPCHECK(hipMemsetAsync(s->d.trace_n, 0, sizeof(uint32_t), s->stream));
+        if (s->mode != GGDMC_FP64_PLAIN) s->prepare_rest(n);
         HIPCHECK(hipEventRecord(s->ev0, s->stream));
         if (s->mode == GGDMC_FP64_PLAIN) s->run_plain(n);
         else s->run_batched(n);
