@@ -748,8 +748,14 @@ void ggdmc_session::set_decomp(bool fused_new, int k_new, int tpt_new, bool chai
     overlap = false;
     if (d.n_phi > 0 && d.S > 0 && !fused && nsweep_sub == 1) {
         const int max_sub = (int)(d.bstats_stride / 4 / ((int64_t)n * d.P)) * kStatBlock;
+        /* the whole-run population step (its LDS holds the whole population
+         * state) runs in these launches only for hyper-only runs, blocked
+         * population sweeps and population migration; without them every
+         * workgroup needs only the per-chain step's or an item's LDS */
+        bool whole_run = d.blocked_pop != 0 || d.pop_mig > 0.0;
+        for (const ggdmc_job &j : jobs) whole_run = whole_run || j.kind == GGDMC_KIND_HYPER;
         ovl_stats_lds = ggdmc_internal_phi_lds(n, d.P, G, 1) <= limit ? 1 : 0;
-        ovl_lds = std::max({sweep_lds, ggdmc_internal_phi_lds(n, d.P, G, ovl_stats_lds),
+        ovl_lds = std::max({sweep_lds, whole_run ? ggdmc_internal_phi_lds(n, d.P, G, ovl_stats_lds) : (size_t)0,
                             ggdmc_internal_phi_chain_lds(d.P, max_sub)});
         overlap = ovl_lds <= limit;
     }
