@@ -48,18 +48,19 @@ W_ALG = {2: 130.0, 4: 255.0}
 # the population workgroups, and the polling of items that wait for their
 # subject's previous iteration): rocprofv3 SQ_INSTS_VALU x 64 / evaluations of
 # a launch of 32 iterations (profiles/subject_sweep_kernel_<workload>.md).
-# Keyed by (workload, chains per item, trials per item, launch mode), for the
+# Keyed by (workload, chains per item, trials per item, launch mode, waves per
+# SIMD of the kernel build), for the
 # decompositions the autotuner picks on these workloads: a figure is reported
 # only for the decomposition it was counted on.
-W_ISSUED = {("c3", 1, 0, 4): 787.6, ("c3", 1, 0, 3): 884.6, ("c3", 2, 0, 4): 663.9,
-            ("c3", 2, 0, 3): 710.4, ("c5", 17, 0, 3): 862.2, ("c5", 17, 0, 4): 862.9,
-            ("c4", 4, 0, 3): 589.5, ("c4", 4, 0, 4): 573.6}
+W_ISSUED = {("c3", 1, 0, 4, 3): 787.6, ("c3", 1, 0, 3, 3): 884.6, ("c3", 2, 0, 4, 3): 663.9,
+            ("c3", 2, 0, 3, 3): 710.4, ("c5", 17, 0, 3, 3): 862.2, ("c5", 17, 0, 4, 3): 862.9,
+            ("c4", 4, 0, 3, 3): 589.5, ("c4", 4, 0, 4, 3): 573.6}
 # HBM bytes per iteration of the subject sweep kernel at those decompositions:
 # rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB (factors of a 256 MiB copy
 # kernel, same files) per launch, divided by the 32 iterations of a launch.
-TRAFFIC = {("c3", 1, 0, 4): 1.572e6, ("c3", 1, 0, 3): 1.301e6, ("c3", 2, 0, 4): 1.311e6,
-           ("c3", 2, 0, 3): 1.282e6, ("c5", 17, 0, 3): 2.167e7, ("c5", 17, 0, 4): 2.206e7,
-           ("c4", 4, 0, 3): 5.486e6, ("c4", 4, 0, 4): 5.524e6}
+TRAFFIC = {("c3", 1, 0, 4, 3): 1.572e6, ("c3", 1, 0, 3, 3): 1.301e6, ("c3", 2, 0, 4, 3): 1.311e6,
+           ("c3", 2, 0, 3, 3): 1.282e6, ("c5", 17, 0, 3, 3): 2.167e7, ("c5", 17, 0, 4, 3): 2.206e7,
+           ("c4", 4, 0, 3, 3): 5.486e6, ("c4", 4, 0, 4, 3): 5.524e6}
 # Iterations a bench session holds at least, whatever --steps asks: the
 # library autotunes the work decomposition of sessions of 64 or more
 # iterations, so the measured configuration never depends on the step count.
@@ -85,9 +86,9 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--profile-iters", type=int, default=32)
     ap.add_argument("--decomp", default="",
-                    help="fixed work items 'k,t[,m]' (chains per item or -1, trials per item or 0, "
-                         "launch mode 1 = a subject launch per chain group, 2 = one for every "
-                         "group) instead of autotuning")
+                    help="fixed work items 'k,t[,m[,w]]' (chains per item or -1, trials per item "
+                         "or 0, launch mode 1 to 4 as make_config's launch_mode, waves per SIMD "
+                         "of the subject kernel 3 or 4) instead of autotuning")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device-to-device copy (counter calibration)")
     return ap.parse_args()
@@ -201,12 +202,15 @@ def main():
     if a.decomp:
         dv = [int(v) for v in a.decomp.split(",")]
         k, t, m = dv[0], dv[1], (dv[2] if len(dv) > 2 else 1)
-        cfg = w.config(nmc=nmc, thin=thin, chains_per_item=k, trials_per_tile=t, launch_mode=m)
+        wps = dv[3] if len(dv) > 3 else 3
+        cfg = w.config(nmc=nmc, thin=thin, chains_per_item=k, trials_per_tile=t, launch_mode=m,
+                       waves_per_simd=wps)
     else:
         cfg = w.config(nmc=nmc, thin=thin)
     sess = g.Session(cfg, w.model, w.jobs)
     kci, tpt, nitems, lmode = sess.decomposition()
-    print("[bench] decomp %d,%d,%d" % (kci, tpt, lmode), file=sys.stderr, flush=True)
+    occ = sess.waves_per_simd()
+    print("[bench] decomp %d,%d,%d (%d waves per SIMD)" % (kci, tpt, lmode, occ), file=sys.stderr, flush=True)
     print("[bench] workload %s: %s; %d subjects, %d trials; items: %s chains x %s trials, "
           "%d per launch, %s; setup %.1fs"
           % (w.name, w.description, w.n_subjects, w.n_trials,
@@ -236,7 +240,7 @@ def main():
     evals_per_launch = (e3 - e2) / max(n_launch, 1)
     iters_per_launch = a.profile_iters / max(n_launch, 1)
     achieved = evals_per_launch * W_ALG[n_acc] / (k_ms / 1e3) / 1e12
-    key = (w.name, kci, tpt, lmode)
+    key = (w.name, kci, tpt, lmode, occ)
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TSLOTS, 2),
                 "unit": "Tslot/s", "frac": round(achieved / VALU_PEAK_TSLOTS, 4),
                 "traffic": TRAFFIC[key] * iters_per_launch if key in TRAFFIC else None,
@@ -312,7 +316,7 @@ def main():
         "config": {"workload": "%s: %s" % (w.name, w.description), "subjects": w.n_subjects,
                    "trials": w.n_trials, "chains": w.nchain, "sweep_groups": 2,
                    "chains_per_item": kci, "trials_per_item": tpt, "items_per_launch": nitems,
-                   "launch_mode": lmode, "runs": a.runs},
+                   "launch_mode": lmode, "waves_per_simd": occ, "runs": a.runs},
         "run_iter_per_s": round(iters_per_s * a.runs, 2),
         "trial_evals_per_s": round(evals_per_s, 1),
         "plain_fp64_iter_per_s": round(plain, 3) if plain else None,

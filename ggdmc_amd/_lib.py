@@ -42,7 +42,8 @@ class Config(C.Structure):
                 ("precision", C.c_int32), ("sweep_groups", C.c_int32),
                 ("trials_per_tile", C.c_int32), ("chains_per_item", C.c_int32),
                 ("launch_mode", C.c_int32), ("iter_offset", C.c_uint64),
-                ("pop_debug", C.c_int32), ("sub_debug", C.c_int32)]
+                ("pop_debug", C.c_int32), ("sub_debug", C.c_int32),
+                ("waves_per_simd", C.c_int32)]
 
 
 class TraceRecord(C.Structure):
@@ -126,6 +127,7 @@ def lib():
                                         C.POINTER(C.c_int32)]
     L.ggdmc_session_trial_evals.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     L.ggdmc_session_decomposition.argtypes = [C.c_void_p, _i32, _i32, _i32, _i32]
+    L.ggdmc_session_waves_per_simd.argtypes = [C.c_void_p, _i32]
     L.ggdmc_session_trace.argtypes = [C.c_void_p, C.POINTER(C.POINTER(TraceRecord)),
                                       C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.ggdmc_last_trace.argtypes = [C.POINTER(C.POINTER(TraceRecord)), C.POINTER(C.c_int64),

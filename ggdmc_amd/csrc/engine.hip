@@ -41,7 +41,7 @@ size_t ggdmc_internal_sweep_lds(int n, int P, int kmax, int per_chain, int chain
                                 int tstage);
 size_t ggdmc_internal_phi_lds(int n, int P, int G, int stats_lds);
 size_t ggdmc_internal_phi_chain_lds(int P, int max_subjects);
-hipError_t ggdmc_internal_launch_sweep(const Dev *d, const SweepArgs *a, int mode, size_t lds,
+hipError_t ggdmc_internal_launch_sweep(const Dev *d, const SweepArgs *a, int mode, size_t lds, int occ,
                                        hipStream_t stream);
 hipError_t ggdmc_internal_launch_phi(const Dev *d, uint32_t iter_off, size_t lds, int stats_lds,
                                      hipStream_t stream);
@@ -162,6 +162,8 @@ struct ggdmc_session {
     bool chained = false;         /* every chain group in one subject launch */
     int ipl = 1;                  /* iterations per subject launch (chained only) */
     bool span = false;            /* chained: an item holds its chain range of every group */
+    int occ = 3;                  /* waves per SIMD of the subject kernel (4: several
+                                     iterations per launch only) */
     bool tiled = false;           /* some subject's trials split over several items */
     int rmax = 1;                 /* chain ranges per group (item_ready stride) */
     int ovl_stats_lds = 0;
@@ -201,7 +203,7 @@ struct ggdmc_session {
 
     void build(const ggdmc_config *c, const ggdmc_model *model, const ggdmc_job *js, int nj);
     void set_decomp(bool fused_new, int k_new, int tpt_new, bool chained_new = false, int ipl_new = 1,
-                    bool span_new = false);
+                    bool span_new = false, int occ_new = 3);
     void default_decomp();
     void enqueue_iteration(uint32_t off, std::vector<hipEvent_t> *ev = nullptr, int count = 1);
     void flush_store();
@@ -609,16 +611,19 @@ void ggdmc_session::default_decomp() {
     const bool one = cfg.launch_mode >= 2;
     const int ipl_cfg = cfg.launch_mode >= 3 ? kItersPerLaunch : 1;
     const bool span_cfg = cfg.launch_mode == 4;
+    require(cfg.waves_per_simd == 0 || cfg.waves_per_simd == 3 || cfg.waves_per_simd == 4,
+            "waves_per_simd must be 0, 3 or 4");
+    const int occ_cfg = cfg.waves_per_simd == 4 && ipl_cfg > 1 ? 4 : 3;
     if (cfg.chains_per_item < 0) {
         require(cfg.trials_per_tile == 0, "one item per subject needs trials_per_tile = 0");
         require(!one, "one item per subject already covers every group in one launch");
         set_decomp(true, 0, 0);
     } else if (cfg.chains_per_item > 0 || cfg.trials_per_tile > 0) {
-        set_decomp(false, cfg.chains_per_item, cfg.trials_per_tile, one, ipl_cfg, span_cfg);
+        set_decomp(false, cfg.chains_per_item, cfg.trials_per_tile, one, ipl_cfg, span_cfg, occ_cfg);
     } else {
         int k = ngmax;
         while (k > 1 && (int64_t)d.S * ((ngmax + k - 1) / k) < 1024) k = (k + 1) / 2;
-        set_decomp(false, k, 0, one, ipl_cfg, span_cfg);
+        set_decomp(false, k, 0, one, ipl_cfg, span_cfg, occ_cfg);
     }
 }
 
@@ -627,7 +632,7 @@ void ggdmc_session::default_decomp() {
  * (rounded up to a multiple of kChunk; 0 = whole subject); or, fused, one
  * item per subject that sweeps every group in one launch. Sizes the LDS. */
 void ggdmc_session::set_decomp(bool fused_new, int k_new, int tpt_new, bool chained_new, int ipl_new,
-                               bool span_new) {
+                               bool span_new, int occ_new) {
     if (tpt_new > 0) tpt_new = ((tpt_new + kChunk - 1) / kChunk) * kChunk;
     require(!fused_new || tpt_new == 0, "one item per subject needs whole subjects");
     require(!(fused_new && chained_new), "one item per subject already covers every group");
@@ -641,6 +646,7 @@ void ggdmc_session::set_decomp(bool fused_new, int k_new, int tpt_new, bool chai
     chained = chained_new;
     ipl = ipl_new;
     span = span_new;
+    occ = ipl_new > 1 ? occ_new : 3;
     tiled = false;
     tpt = tpt_new;
     d.tpt = tpt;
@@ -797,7 +803,7 @@ void ggdmc_session::enqueue_iteration(uint32_t off, std::vector<hipEvent_t> *ev,
                 ev->push_back(e1);
                 HIPCHECK(hipEventRecord(e0, stream));
             }
-            HIPCHECK(ggdmc_internal_launch_sweep(&d, &a, lk, with_phi ? ovl_lds : sweep_lds, stream));
+            HIPCHECK(ggdmc_internal_launch_sweep(&d, &a, lk, with_phi ? ovl_lds : sweep_lds, occ, stream));
             if (ev) HIPCHECK(hipEventRecord(e1, stream));
         }
     }
@@ -910,9 +916,10 @@ void ggdmc_session::autotune() {
         bool chained;
         int ipl;
         bool span;
+        int occ;
     };
     const int ngmax = (d.nchain + d.G - 1) / d.G;
-    std::vector<Cand> cands{{true, 0, 0, false, 1, false}};
+    std::vector<Cand> cands{{true, 0, 0, false, 1, false, 3}};
     std::vector<int> ks;
     for (int k : {1, 2, 4, 8, ngmax})
         if (k <= ngmax && std::find(ks.begin(), ks.end(), k) == ks.end()) ks.push_back(k);
@@ -923,14 +930,18 @@ void ggdmc_session::autotune() {
                 if (t == 0 || t < max_ntr) {
                     const int64_t nt = t == 0 ? 1 : (max_ntr + t - 1) / t;
                     if ((int64_t)d.S * ((ngmax + k - 1) / k) * nt <= ((int64_t)1 << 22)) {
-                        cands.push_back({false, k, t, one == 1, 1, false});
+                        cands.push_back({false, k, t, one == 1, 1, false, 3});
                         /* several iterations per launch: whole subjects, the
                          * population step inside the launch; items of one
                          * group, or spanning every group */
                         if (one == 1 && t == 0 && cfg.launch_mode != 2) {
-                            cands.push_back({false, k, t, true, kItersPerLaunch, false});
-                            if (span_fits((ngmax + k - 1) / k))
-                                cands.push_back({false, k, t, true, kItersPerLaunch, true});
+                            /* the subject kernel at 3 and at 4 waves per SIMD */
+                            for (int oc : {3, 4}) {
+                                if (cfg.waves_per_simd != 0 && oc != cfg.waves_per_simd) continue;
+                                cands.push_back({false, k, t, true, kItersPerLaunch, false, oc});
+                                if (span_fits((ngmax + k - 1) / k))
+                                    cands.push_back({false, k, t, true, kItersPerLaunch, true, oc});
+                            }
                         }
                     }
                 }
@@ -982,7 +993,7 @@ void ggdmc_session::autotune() {
     const int K = last_iter >= 3 * (uint32_t)kItersPerLaunch ? kItersPerLaunch : 16;
     for (const Cand &c : cands) {
         try {
-            set_decomp(c.fused, c.k, c.tpt, c.chained, c.ipl, c.span);
+            set_decomp(c.fused, c.k, c.tpt, c.chained, c.ipl, c.span, c.occ);
         } catch (const Error &) {
             if (c.ipl == 1) throw;
             continue;   /* no population step inside the launch for this design */
@@ -1009,7 +1020,7 @@ void ggdmc_session::autotune() {
         }
     }
     next_iter = it0;
-    set_decomp(pick.fused, pick.k, pick.tpt, pick.chained, pick.ipl, pick.span);
+    set_decomp(pick.fused, pick.k, pick.tpt, pick.chained, pick.ipl, pick.span, pick.occ);
 }
 
 static int fail(const std::exception &e) {
@@ -1136,6 +1147,12 @@ int ggdmc_session_advance(ggdmc_session *s, int32_t n_iter) {
 int ggdmc_session_elapsed_ms(ggdmc_session *s, float *ms) {
     if (!s || !ms) return invalid("session / ms is null");
     *ms = s->last_ms;
+    return GGDMC_OK;
+}
+
+int ggdmc_session_waves_per_simd(ggdmc_session *s, int32_t *waves_per_simd) {
+    if (!s || !waves_per_simd) return invalid("session / waves_per_simd is null");
+    *waves_per_simd = s->occ;
     return GGDMC_OK;
 }
 

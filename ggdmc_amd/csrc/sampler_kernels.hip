@@ -1687,8 +1687,10 @@ __device__ __forceinline__ void late_prior(const Dev &d, const JobDev &job, cons
 }
 
 /* ------------------------------------------------------------ sweep kernel */
-template <int MODE, bool MULTI>
-__global__ __launch_bounds__(kBlock, 3) void subject_sweep_kernel(Dev d, SweepArgs a) {
+/* OCC: waves per SIMD the kernel is built for (3: up to 168 VGPRs a lane, 4:
+ * 128, with values of the cold paths in scratch). */
+template <int MODE, bool MULTI, int OCC>
+__global__ __launch_bounds__(kBlock, OCC) void subject_sweep_kernel(Dev d, SweepArgs a) {
     extern __shared__ __attribute__((aligned(16))) char lds_raw[];
     /* several iterations per launch (MULTI): segment seg of seg_wgs
      * workgroups runs iteration base + seg; its workgroups wait only on
@@ -2280,22 +2282,31 @@ size_t ggdmc_internal_phi_chain_lds(int P, int max_subjects) {
     return phi_chain_lds_bytes(P, (max_subjects + kStatBlock - 1) / kStatBlock);
 }
 
-hipError_t ggdmc_internal_launch_sweep(const Dev *d, const SweepArgs *a, int mode, size_t lds,
+/* occ (waves per SIMD) 4 exists for several iterations per launch only. */
+hipError_t ggdmc_internal_launch_sweep(const Dev *d, const SweepArgs *a, int mode, size_t lds, int occ,
                                        hipStream_t stream) {
     const dim3 block(kBlock);
+    const bool mixed = mode == GGDMC_MIXED_BATCHED;
     if (a->iters > 1) {
         const dim3 grid(a->iters * a->seg_wgs);
-        if (mode == GGDMC_MIXED_BATCHED)
-            subject_sweep_kernel<MODE_MIXED, true><<<grid, block, lds, stream>>>(*d, *a);
-        else
-            subject_sweep_kernel<MODE_NAIVE, true><<<grid, block, lds, stream>>>(*d, *a);
+        if (occ == 4) {
+            if (mixed)
+                subject_sweep_kernel<MODE_MIXED, true, 4><<<grid, block, lds, stream>>>(*d, *a);
+            else
+                subject_sweep_kernel<MODE_NAIVE, true, 4><<<grid, block, lds, stream>>>(*d, *a);
+        } else {
+            if (mixed)
+                subject_sweep_kernel<MODE_MIXED, true, 3><<<grid, block, lds, stream>>>(*d, *a);
+            else
+                subject_sweep_kernel<MODE_NAIVE, true, 3><<<grid, block, lds, stream>>>(*d, *a);
+        }
         return hipGetLastError();
     }
     const dim3 grid(a->phi_wgs + a->n_items);
-    if (mode == GGDMC_MIXED_BATCHED)
-        subject_sweep_kernel<MODE_MIXED, false><<<grid, block, lds, stream>>>(*d, *a);
+    if (mixed)
+        subject_sweep_kernel<MODE_MIXED, false, 3><<<grid, block, lds, stream>>>(*d, *a);
     else
-        subject_sweep_kernel<MODE_NAIVE, false><<<grid, block, lds, stream>>>(*d, *a);
+        subject_sweep_kernel<MODE_NAIVE, false, 3><<<grid, block, lds, stream>>>(*d, *a);
     return hipGetLastError();
 }
 

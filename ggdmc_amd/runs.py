@@ -149,19 +149,21 @@ def make_config(nmc, nchain, thin=1, pop_migration_prob=0.0, sub_migration_prob=
                 gamma_precursor=2.38, rp=0.001, is_hblocked=False, is_pblocked=False,
                 precision=_lib.MIXED_BATCHED, sweep_groups=0, trials_per_tile=0,
                 chains_per_item=0, launch_mode=0, iter_offset=0, pop_debug=False,
-                sub_debug=False):
+                sub_debug=False, waves_per_simd=0):
     """ggdmc_config. trials_per_tile / chains_per_item / launch_mode 0 let the
     library autotune the work split; launch_mode 1 = a subject launch per chain
     group, 2 = one for every group, 3 = one for every group and 32 iterations,
     4 = as 3 with an item holding its chain range of every group.
-    None of them changes a result bit. pop_debug / sub_debug record the
-    decisions of the population level / of the subjects (Session.trace)."""
+    waves_per_simd (0 = autotuned, 3 or 4) picks the build of the subject kernel
+    for launch modes 3 and 4. None of them changes a result bit. pop_debug /
+    sub_debug record the decisions of the population level / of the subjects
+    (Session.trace)."""
     return _lib.Config(int(nmc), int(thin), int(nchain), float(pop_migration_prob),
                        float(sub_migration_prob), float(gamma_precursor), float(rp),
                        int(bool(is_hblocked)), int(bool(is_pblocked)), int(precision),
                        int(sweep_groups), int(trials_per_tile), int(chains_per_item),
                        int(launch_mode), int(iter_offset), int(bool(pop_debug)),
-                       int(bool(sub_debug)))
+                       int(bool(sub_debug)), int(waves_per_simd))
 
 
 def format_trace(records):
@@ -296,8 +298,8 @@ class Session:
 
     def decomposition(self):
         """(chains per item or -1 for one item per subject, trials per item or 0
-        for whole subjects, items of the largest launch, launch mode: 1 = one
-        subject launch per chain group, 2 = one for every group)."""
+        for whole subjects, items of the largest launch, launch mode 1 to 4 as
+        make_config's launch_mode)."""
         k = C.c_int32()
         t = C.c_int32()
         m = C.c_int32()
@@ -305,6 +307,12 @@ class Session:
         check(_lib.lib().ggdmc_session_decomposition(self.h, C.byref(k), C.byref(t), C.byref(m),
                                                      C.byref(lm)))
         return k.value, t.value, m.value, lm.value
+
+    def waves_per_simd(self):
+        """Waves per SIMD the subject kernel of this session was built for (3 or 4)."""
+        w = C.c_int32()
+        check(_lib.lib().ggdmc_session_waves_per_simd(self.h, C.byref(w)))
+        return w.value
 
     def download(self):
         """Returns per job {'phi': Posterior|None, 'subjects': [Posterior]}."""

@@ -78,6 +78,10 @@ typedef struct {
     int32_t sub_debug;      /* de_input@sub_debug: record the decisions of single-subject and
                                hyper-only runs and of the first three subjects of hierarchical
                                runs (src/de.cpp:85-106, :327) */
+    int32_t waves_per_simd; /* launch modes 3 and 4: the subject kernel built for 3 waves per
+                               SIMD (up to 168 registers a lane) or 4 (128, some values kept
+                               in scratch); 0 = autotuned with the decomposition, else 3;
+                               never changes results */
 } ggdmc_config;
 
 /* One Metropolis decision of a traced level (ggdmc_config pop_debug /
@@ -215,6 +219,10 @@ void ggdmc_session_destroy(ggdmc_session *s);
 int ggdmc_session_decomposition(ggdmc_session *s, int32_t *chains_per_item,
                                 int32_t *trials_per_tile, int32_t *items_per_launch,
                                 int32_t *launch_mode);
+
+/* Waves per SIMD the session's subject kernel was built for (3 or 4, as
+ * ggdmc_config.waves_per_simd). No counterpart in the reference. */
+int ggdmc_session_waves_per_simd(ggdmc_session *s, int32_t *waves_per_simd);
 
 /* Decisions recorded during the last advance of a session (pop_debug /
  * sub_debug), in the reference's print order: by run, iteration, level

@@ -179,23 +179,26 @@ def test_decomposition_never_changes_results(gpu, precision):
     # group, 2 = every group in one launch, later groups waiting for earlier,
     # 3 = as 2 with several iterations per launch, 4 = as 3 with an item
     # holding its chain range of every group)
-    for k, tpt, lm in ((-1, 0, 1), (0, 0, 1), (1, 0, 1), (4, 128, 1), (2, 256, 1), (0, 384, 1),
-                       (0, 0, 2), (1, 0, 2), (4, 128, 2), (2, 256, 2), (0, 0, 3), (1, 0, 3),
-                       (2, 0, 3), (4, 128, 3), (0, 0, 4), (1, 0, 4), (2, 0, 4), (5, 0, 4)):
+    # (the fourth entry: the subject kernel built for that many waves per SIMD)
+    for k, tpt, lm, w in ((-1, 0, 1, 3), (0, 0, 1, 3), (1, 0, 1, 3), (4, 128, 1, 3), (2, 256, 1, 3),
+                          (0, 384, 1, 3), (0, 0, 2, 3), (1, 0, 2, 3), (4, 128, 2, 3), (2, 256, 2, 3),
+                          (0, 0, 3, 3), (1, 0, 3, 3), (2, 0, 3, 3), (4, 128, 3, 3), (0, 0, 4, 3),
+                          (1, 0, 4, 3), (2, 0, 4, 3), (5, 0, 4, 3), (1, 0, 3, 4), (2, 0, 4, 4),
+                          (4, 128, 3, 4)):
         cfg = gpu.make_config(nmc=6, nchain=n, pop_migration_prob=0.2, sub_migration_prob=0.3,
                               precision=prec, sweep_groups=2, chains_per_item=k,
-                              trials_per_tile=tpt, launch_mode=lm)
+                              trials_per_tile=tpt, launch_mode=lm, waves_per_simd=w)
         r = gpu.run(cfg, fix.model, fix.pop_prior, fix.hyper_prior, datas, (phi, plp, pll), subs,
                     seed=31)
         if ref is None:
             ref = r
             continue
-        assert np.array_equal(r["phi"].theta, ref["phi"].theta), (k, tpt, lm)
-        assert np.array_equal(r["phi"].log_likelihoods, ref["phi"].log_likelihoods), (k, tpt, lm)
+        assert np.array_equal(r["phi"].theta, ref["phi"].theta), (k, tpt, lm, w)
+        assert np.array_equal(r["phi"].log_likelihoods, ref["phi"].log_likelihoods), (k, tpt, lm, w)
         for a, b in zip(r["subject_theta"], ref["subject_theta"]):
-            assert np.array_equal(a.theta, b.theta), (k, tpt, lm)
-            assert np.array_equal(a.log_likelihoods, b.log_likelihoods), (k, tpt, lm)
-            assert np.array_equal(a.summed_log_prior, b.summed_log_prior), (k, tpt, lm)
+            assert np.array_equal(a.theta, b.theta), (k, tpt, lm, w)
+            assert np.array_equal(a.log_likelihoods, b.log_likelihoods), (k, tpt, lm, w)
+            assert np.array_equal(a.summed_log_prior, b.summed_log_prior), (k, tpt, lm, w)
 
 
 def test_loglike_api_equals_sampler_values(gpu):

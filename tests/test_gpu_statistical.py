@@ -126,10 +126,11 @@ def test_mixed_batch_schedules_agree(gpu):
     # same bits (the waits between groups only order the work)
     w = workloads.make("c4", scale=1 / 16, ll_fn=_oracle_ll)
     ref = None
-    for k, tpt, lm in ((0, 0, 1), (2, 0, 2), (0, 256, 2), (4, 128, 2), (2, 0, 3), (0, 256, 3),
-                       (1, 0, 4), (4, 0, 4)):
+    for k, tpt, lm, wps in ((0, 0, 1, 3), (2, 0, 2, 3), (0, 256, 2, 3), (4, 128, 2, 3), (2, 0, 3, 3),
+                            (0, 256, 3, 3), (1, 0, 4, 3), (4, 0, 4, 3), (4, 0, 3, 4), (2, 0, 4, 4)):
         cfg = gpu.make_config(nmc=6, nchain=w.nchain, precision=gpu.MIXED_BATCHED, sweep_groups=2,
                               chains_per_item=k, trials_per_tile=tpt, launch_mode=lm,
+                              waves_per_simd=wps,
                               sub_migration_prob=0.2, pop_migration_prob=0.2)
         res = gpu.run_batch(cfg, w.model, w.jobs)
         if ref is None:
