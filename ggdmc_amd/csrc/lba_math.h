@@ -230,8 +230,9 @@ GGDMC_FN double mixed_winner_logpdf(const AccK &k, double rt, bool &ill) {
     const bool pos = t > 0.0;
     const Core c = mixed_core<Ops>(k, pos ? t : 1.0);
     const double num = k.v * c.dP + k.s * c.dphi;
-    const double err = kTailErr * std::fabs(k.v) * (c.q1 + c.q2) + kExpErr * k.s * (c.e1 + c.e2) * kInvSqrt2Pi +
-                       (c.z2 > 0.0 ? 2.0 * kUlp1 * std::fabs(k.v) : 0.0);
+    /* |v| (kTailErr (q1 + q2) + [z2 > 0] 2 ulp) + s (e1 + e2) kExpErr / sqrt(2 pi) */
+    const double tv = std::fma(kTailErr, c.q1 + c.q2, c.z2 > 0.0 ? 2.0 * kUlp1 : 0.0);
+    const double err = std::fma(std::fabs(k.v), tv, k.s * (kExpErr * kInvSqrt2Pi) * (c.e1 + c.e2));
     const double pdf = num * k.invA * k.inv_den;
     /* both floored unless the pdf or its uncertainty reaches the floor */
     ill |= pos && err > kCondTol * std::fabs(num) && (std::fabs(num) + err) * k.invA * k.inv_den > 0.5 * kFloor;
@@ -257,9 +258,10 @@ GGDMC_FN double mixed_loser_survivor(const AccK &k, double rt, bool &ill) {
     const double S0 = 1.0 - (F > kFloor ? F : (F == F ? kFloor : F));
     /* error of F from the tails, plus the reference's own noise from Phi
      * values rounded near 1 (z > 0), against the survivor */
-    const double errF = (kExpErr * c.zs * (c.e1 + c.e2) * kInvSqrt2Pi + kTailErr * (achi * c.q1 + achiA * c.q2) +
-                         kUlp1 * ((c.z1 > 0.0 ? achi : 0.0) + (c.z2 > 0.0 ? achiA : 0.0))) *
-                        k.invA * k.inv_den;
+    const double u1 = std::fma(kTailErr, c.q1, c.z1 > 0.0 ? kUlp1 : 0.0);
+    const double u2 = std::fma(kTailErr, c.q2, c.z2 > 0.0 ? kUlp1 : 0.0);
+    const double errF =
+        std::fma(achi, u1, std::fma(achiA, u2, c.zs * (kExpErr * kInvSqrt2Pi) * (c.e1 + c.e2))) * k.invA * k.inv_den;
     /* below a hundredth of an ulp of 1 the reference's own rounding of F is
      * reproduced exactly, whatever the survivor */
     ill |= pos && errF > kCondTol * std::fabs(S0) && errF > 1e-18;
