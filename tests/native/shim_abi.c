@@ -1,5 +1,5 @@
 /*
- * The ABI side of the Rcpp shim in plain C, for tests: what
+ * The ABI side of the Rcpp shim that replaces src/de2R.cpp, in plain C, for tests: what
  * the shim's run_subject / run_hyper / run do with R's buffers before and
  * while calling libggdmc_hip, without R.
  *
