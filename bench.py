@@ -52,17 +52,18 @@ W_ALG = {2: 130.0, 4: 255.0}
 # SIMD of the kernel build), for the
 # decompositions the autotuner picks on these workloads: a figure is reported
 # only for the decomposition it was counted on.
-W_ISSUED = {("c3", 1, 0, 4, 3): 787.6, ("c3", 1, 0, 3, 3): 884.6, ("c3", 2, 0, 4, 3): 663.9,
-            ("c3", 2, 0, 3, 3): 710.4, ("c5", 17, 0, 3, 3): 862.2, ("c5", 17, 0, 4, 3): 862.9,
-            ("c4", 4, 0, 3, 3): 589.5, ("c4", 4, 0, 4, 3): 573.6, ("c4", 4, 0, 3, 4): 626.3,
-            ("c5", 17, 0, 3, 4): 863.6}
+W_ISSUED = {('c3', 1, 0, 4, 3): 784.2, ('c3', 1, 0, 3, 3): 880.9, ('c3', 2, 0, 4, 3): 661.5,
+            ('c3', 2, 0, 3, 3): 706.7, ('c4', 4, 0, 3, 4): 622.7, ('c4', 4, 0, 3, 3): 586.1,
+            ('c4', 4, 0, 4, 3): 570.5, ('c5', 17, 0, 3, 4): 858.6, ('c5', 17, 0, 3, 3): 857.3,
+            ('c5', 17, 0, 4, 3): 857.7}
 # HBM bytes per iteration of the subject sweep kernel at those decompositions:
 # rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB (factors of a 256 MiB copy
 # kernel, same files) per launch, divided by the 32 iterations of a launch.
-TRAFFIC = {("c3", 1, 0, 4, 3): 1.572e6, ("c3", 1, 0, 3, 3): 1.301e6, ("c3", 2, 0, 4, 3): 1.311e6,
-           ("c3", 2, 0, 3, 3): 1.282e6, ("c5", 17, 0, 3, 3): 2.167e7, ("c5", 17, 0, 4, 3): 2.206e7,
-           ("c4", 4, 0, 3, 3): 5.486e6, ("c4", 4, 0, 4, 3): 5.524e6, ("c4", 4, 0, 3, 4): 8.153e6,
-           ("c5", 17, 0, 3, 4): 2.614e7}
+TRAFFIC = {('c3', 1, 0, 4, 3): 1.564e+06, ('c3', 1, 0, 3, 3): 1.291e+06,
+           ('c3', 2, 0, 4, 3): 1.315e+06, ('c3', 2, 0, 3, 3): 1.275e+06,
+           ('c4', 4, 0, 3, 4): 8.186e+06, ('c4', 4, 0, 3, 3): 5.486e+06,
+           ('c4', 4, 0, 4, 3): 5.531e+06, ('c5', 17, 0, 3, 4): 2.573e+07,
+           ('c5', 17, 0, 3, 3): 2.161e+07, ('c5', 17, 0, 4, 3): 2.203e+07}
 # Iterations a bench session holds at least, whatever --steps asks: the
 # library autotunes the work decomposition of sessions of 64 or more
 # iterations, so the measured configuration never depends on the step count.
