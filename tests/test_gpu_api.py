@@ -73,6 +73,24 @@ def test_errors_come_back_as_exceptions(gpu):
                         job.start, seed=1)
 
 
+def test_session_reports_its_decomposition_and_kernel_build(gpu):
+    # the decomposition and kernel build a configuration asks for are the ones
+    # the session runs; the 4-wave build exists for several iterations per
+    # launch only, and a value other than 0, 3 or 4 is refused
+    w = workloads.make("c3", scale=1 / 8, ll_fn=lambda m, d, th: np.zeros(len(th)))
+    for k, lm, wps, expect in ((2, 3, 4, 4), (1, 4, 4, 4), (2, 4, 3, 3), (2, 1, 4, 3)):
+        cfg = w.config(nmc=4, chains_per_item=k, launch_mode=lm, waves_per_simd=wps)
+        s = gpu.Session(cfg, w.model, w.jobs)
+        try:
+            kci, tpt, nitems, mode = s.decomposition()
+            assert (kci, tpt, mode) == (k, 0, lm)
+            assert s.waves_per_simd() == expect
+        finally:
+            s.close()
+    with pytest.raises(gpu.GgdmcError, match="waves_per_simd"):
+        gpu.Session(w.config(nmc=4, waves_per_simd=2), w.model, w.jobs)
+
+
 def _trace_run(gpu, cfg, model, jobs):
     s = gpu.Session(cfg, model, jobs)
     try:
