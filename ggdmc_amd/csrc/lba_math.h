@@ -27,8 +27,10 @@
  *        Q(z)*exp(z^2/2), approximated as t*sum_k c_k t^k with t = 1/(1+0.275|z|)
  *        (coefficients fitted for this library by weighted least squares to
  *        0.5*erfcx(z/sqrt(2)) on [0, 14]; max relative error 8.2e-10);
- *      * the small tail of Phi is always the one computed, so
- *        Phi(z1) - Phi(z2) is taken as Q(z2) - Q(z1) when z2 >= 0;
+ *      * the small tail of Phi is always the one computed; the winner's
+ *        Phi(z1) - Phi(z2) for z2 > 0 is the reference's own FP64 value
+ *        RN(1 - Q(z1)) - RN(1 - Q(z2)), reproduced from the computed tails
+ *        wherever their error cannot move either rounding;
  *      * the survivor of a loser is formed from whichever of the untruncated
  *        CDF G or its complement 1 - G is small (both written without
  *        cancellation, see below) and then passed through the reference's own
@@ -204,9 +206,10 @@ GGDMC_FN void naive_pdf_cdf(const AccK &k, double t, double &pdf, double &cdf) {
 /* Error model of the mixed evaluation: every computed tail and exp(-z^2/2)
  * carries a relative error of up to kTailErr (FP32 exp2); a result whose
  * bound exceeds kCondTol relative, or whose reference FP64 value carries more
- * rounding noise than that (Phi rounded near 1, ulp 1.1e-16, scaled by the
- * formula's coefficients), is flagged ill-conditioned and the trial is
- * evaluated with the reference's FP64 formula instead. */
+ * rounding noise than that where the rounding is not reproduced (Phi rounded
+ * near 1, ulp 1.1e-16, scaled by the formula's coefficients), is flagged
+ * ill-conditioned and the trial is evaluated with the reference's FP64
+ * formula instead. */
 constexpr double kTailErr = 1e-9;   /* Mills-ratio fit (8.2e-10) */
 constexpr double kExpErr = 1e-15;   /* exp(-z^2/2) in FP64 arithmetic */
 constexpr double kCondTol = 1e-5;
@@ -229,9 +232,23 @@ GGDMC_FN double mixed_winner_logpdf(const AccK &k, double rt, bool &ill) {
     const double t = rt - k.t0;
     const bool pos = t > 0.0;
     const Core c = mixed_core<Ops>(k, pos ? t : 1.0);
-    const double num = k.v * c.dP + k.s * c.dphi;
-    /* |v| (kTailErr (q1 + q2) + [z2 > 0] 2 ulp) + s (e1 + e2) kExpErr / sqrt(2 pi) */
-    const double tv = std::fma(kTailErr, c.q1 + c.q2, c.z2 > 0.0 ? 2.0 * kUlp1 : 0.0);
+    /* z2 > 0: Phi(z1) - Phi(z2) as the reference rounds it, RN(1 - Q1) -
+     * RN(1 - Q2) (an exact difference). 1 - q = p + r exactly (Fast2Sum,
+     * p = RN(1 - q)); when r lies further than q's error from the rounding
+     * midpoint p -+ ulp/2 (ulp = 2^-53 on [1/2, 1]), the reference's 1 - Q
+     * rounds to p as well and its quantised difference is reproduced exactly;
+     * otherwise it may be one ulp off. In the far right tail that
+     * quantisation is most of the reference's v * (Phi(z1) - Phi(z2)). */
+    const bool right = c.z2 > 0.0;
+    const double p1 = 1.0 - c.q1, p2 = 1.0 - c.q2;
+    const double r1 = -c.q1 - (p1 - 1.0), r2 = -c.q2 - (p2 - 1.0);
+    const double eq1 = kTailErr * c.q1, eq2 = kTailErr * c.q2;
+    const double dP = right ? p1 - p2 : c.dP;
+    /* |v| (error of dP + 4 ulp of it for the products) + s (e1 + e2) kExpErr / sqrt(2 pi) */
+    const double tv = right ? ((0.5 * kUlp1 - std::fabs(r1) > eq1 ? 0.0 : eq1 + kUlp1) +
+                               (0.5 * kUlp1 - std::fabs(r2) > eq2 ? 0.0 : eq2 + kUlp1) + 4.0 * kUlp1 * dP)
+                            : eq1 + eq2;
+    const double num = k.v * dP + k.s * c.dphi;
     const double err = std::fma(std::fabs(k.v), tv, k.s * (kExpErr * kInvSqrt2Pi) * (c.e1 + c.e2));
     const double pdf = num * k.invA * k.inv_den;
     /* both floored unless the pdf or its uncertainty reaches the floor */
