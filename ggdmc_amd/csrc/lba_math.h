@@ -27,10 +27,12 @@
  *        Q(z)*exp(z^2/2), approximated as t*sum_k c_k t^k with t = 1/(1+0.275|z|)
  *        (coefficients fitted for this library by weighted least squares to
  *        0.5*erfcx(z/sqrt(2)) on [0, 14]; max relative error 8.2e-10);
- *      * the small tail of Phi is always the one computed; the winner's
- *        Phi(z1) - Phi(z2) for z2 > 0 is the reference's own FP64 value
- *        RN(1 - Q(z1)) - RN(1 - Q(z2)), reproduced from the computed tails
- *        wherever their error cannot move either rounding;
+ *      * the small tail of Phi is always the one computed, so
+ *        Phi(z1) - Phi(z2) is taken as Q(z2) - Q(z1) when z2 >= 0; where
+ *        the reference's own FP64 value RN(1 - Q(z1)) - RN(1 - Q(z2)) could
+ *        differ from that by more than the tolerance (the winner's far right
+ *        tail), that rounded difference is reproduced from the computed
+ *        tails whenever their error cannot move either rounding;
  *      * the survivor of a loser is formed from whichever of the untruncated
  *        CDF G or its complement 1 - G is small (both written without
  *        cancellation, see below) and then passed through the reference's own
@@ -232,24 +234,29 @@ GGDMC_FN double mixed_winner_logpdf(const AccK &k, double rt, bool &ill) {
     const double t = rt - k.t0;
     const bool pos = t > 0.0;
     const Core c = mixed_core<Ops>(k, pos ? t : 1.0);
-    /* z2 > 0: Phi(z1) - Phi(z2) as the reference rounds it, RN(1 - Q1) -
-     * RN(1 - Q2) (an exact difference). 1 - q = p + r exactly (Fast2Sum,
-     * p = RN(1 - q)); when r lies further than q's error from the rounding
-     * midpoint p -+ ulp/2 (ulp = 2^-53 on [1/2, 1]), the reference's 1 - Q
-     * rounds to p as well and its quantised difference is reproduced exactly;
-     * otherwise it may be one ulp off. In the far right tail that
-     * quantisation is most of the reference's v * (Phi(z1) - Phi(z2)). */
-    const bool right = c.z2 > 0.0;
-    const double p1 = 1.0 - c.q1, p2 = 1.0 - c.q2;
-    const double r1 = -c.q1 - (p1 - 1.0), r2 = -c.q2 - (p2 - 1.0);
-    const double eq1 = kTailErr * c.q1, eq2 = kTailErr * c.q2;
-    const double dP = right ? p1 - p2 : c.dP;
-    /* |v| (error of dP + 4 ulp of it for the products) + s (e1 + e2) kExpErr / sqrt(2 pi) */
-    const double tv = right ? ((0.5 * kUlp1 - std::fabs(r1) > eq1 ? 0.0 : eq1 + kUlp1) +
-                               (0.5 * kUlp1 - std::fabs(r2) > eq2 ? 0.0 : eq2 + kUlp1) + 4.0 * kUlp1 * dP)
-                            : eq1 + eq2;
-    const double num = k.v * dP + k.s * c.dphi;
-    const double err = std::fma(std::fabs(k.v), tv, k.s * (kExpErr * kInvSqrt2Pi) * (c.e1 + c.e2));
+    /* |v| (kTailErr (q1 + q2) + [z2 > 0] 2 ulp) + s (e1 + e2) kExpErr / sqrt(2 pi) */
+    const double tv = std::fma(kTailErr, c.q1 + c.q2, c.z2 > 0.0 ? 2.0 * kUlp1 : 0.0);
+    const double ex = k.s * (kExpErr * kInvSqrt2Pi) * (c.e1 + c.e2);
+    double num = k.v * c.dP + k.s * c.dphi;
+    double err = std::fma(std::fabs(k.v), tv, ex);
+    if (__builtin_expect(c.z2 > 0.0 && err > kCondTol * std::fabs(num), 0)) {
+        /* the far right tail, where the reference's Phi(z1) - Phi(z2) is the
+         * difference of two values rounded near 1: take that difference,
+         * RN(1 - Q1) - RN(1 - Q2), itself. 1 - q = p + r exactly (Fast2Sum,
+         * p = RN(1 - q)); when r lies further than q's error from the rounding
+         * midpoint p -+ ulp/2 (ulp = 2^-53 on [1/2, 1]), the reference's 1 - Q
+         * rounds to p as well and the quantised difference is reproduced
+         * exactly; otherwise it may be one ulp off */
+        const double p1 = 1.0 - c.q1, p2 = 1.0 - c.q2;
+        const double r1 = -c.q1 - (p1 - 1.0), r2 = -c.q2 - (p2 - 1.0);
+        const double eq1 = kTailErr * c.q1, eq2 = kTailErr * c.q2;
+        const double dP = p1 - p2;
+        /* |v| (error of dP + 4 ulp of it for the products) + the exp term */
+        const double tr = (0.5 * kUlp1 - std::fabs(r1) > eq1 ? 0.0 : eq1 + kUlp1) +
+                          (0.5 * kUlp1 - std::fabs(r2) > eq2 ? 0.0 : eq2 + kUlp1) + 4.0 * kUlp1 * dP;
+        num = k.v * dP + k.s * c.dphi;
+        err = std::fma(std::fabs(k.v), tr, ex);
+    }
     const double pdf = num * k.invA * k.inv_den;
     /* both floored unless the pdf or its uncertainty reaches the floor */
     ill |= pos && err > kCondTol * std::fabs(num) && (std::fabs(num) + err) * k.invA * k.inv_den > 0.5 * kFloor;
