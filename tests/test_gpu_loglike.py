@@ -86,3 +86,32 @@ def test_floored_and_empty_and_nan(gpu):
     bad[0] = np.nan
     for prec in (gpu.FP64_PLAIN, gpu.MIXED_BATCHED):
         assert np.isnan(gpu.loglike(fix.model, fix.sub_data, bad[None, :], prec)[0])
+
+
+def test_mixed_far_tail_winner_trials(gpu):
+    # fast responses of a winner far from its threshold (z2 = 5 .. 7), the
+    # density near the 1e-10 floor: there the reference's v * (Phi(z1) -
+    # Phi(z2)) is quantised by its own rounding near 1, which the batched path
+    # reproduces or evaluates with the reference formula. One trial of cell
+    # s1.r1 (r1 wins) under 4096 parameter vectors, each term against the oracle
+    from ggdmc_amd.model import build_model
+    m = build_model(p_map={"A": ["1"], "B": ["1"], "t0": ["1"], "mean_v": ["M"], "sd_v": ["M"],
+                           "st0": ["1"]}, match_map={"M": {"s1": "r1", "s2": "r2"}},
+                    factors={"S": ["s1", "s2"]}, constants={"st0": 0.0, "sd_v.false": 0.3},
+                    accumulators=["r1", "r2"])
+    assert m.pnames == ["A", "B", "mean_v.false", "mean_v.true", "sd_v.true", "t0"]
+    data = gpu.SubjectData.from_cells([[0.35], [], [], []])
+    rng = np.random.default_rng(8)
+    n = 4096
+    A, v, s = rng.uniform(0.2, 1.0, n), rng.uniform(3, 12, n), rng.uniform(0.5, 1.5, n)
+    t = 0.35 - 0.2
+    B = t * (v + rng.uniform(5.0, 7.0, n) * s)          # z2 = B / (t s) - v / s
+    th = np.stack([A, B, np.full(n, 1.0), v, s, np.full(n, 0.2)], axis=1)
+    got = gpu.loglike(m, data, th, gpu.MIXED_BATCHED)
+    om = O.Model(m.winner, m.par_src, m.par_const, m.npar)
+    ref = np.array([O.lba_loglike(om, data.rt, data.cell_off, x) for x in th])
+    far = (ref > np.log(2e-10)) & (ref < np.log(1e-6))
+    d = np.abs(got - ref)
+    print("far-tail terms: %d, max |dLL| %.2e" % (far.sum(), d[far].max()))
+    assert far.sum() > 2000
+    assert d.max() <= 1e-5
