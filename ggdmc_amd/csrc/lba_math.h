@@ -98,16 +98,23 @@ GGDMC_FN double rcp64(double x) {
     return r * (2.0 - x * r);
 }
 
+/* rcp64 for x >= 1 (or NaN): no clamp of the FP32 seed needed. */
+template <class Ops>
+GGDMC_FN double rcp64_ge1(double x) {
+    const double r = (double)Ops::rcp((float)x);
+    return r * (2.0 - x * r);
+}
+
 /* 2^y for y <= 0 in FP64 arithmetic: y = n + j/64 + f with integers n, j
  * (0 <= j < 64) and |f| <= 1/128; 2^(j/64) from a table of correctly rounded
  * values (Ops::exp2_frac), 2^f = exp(f ln 2) from its Taylor series to degree
- * 5 (truncation below 4e-17 for |f ln 2| <= 0.0055). Underflows to 0 below
- * 2^-1074; NaN propagates. */
+ * 5 (truncation below 4e-17 for |f ln 2| <= 0.0055). y <= -1075 (and -inf)
+ * is taken as -1075, whose 2^-1075 rounds to 0; a NaN y also gives 0 (fmax),
+ * so callers carry NaN through another factor (mixed_tail: the Mills ratio
+ * of the same argument). */
 template <class Ops>
 GGDMC_FN double exp2_neg(double y) {
-    /* branch-free (selects), so that independent evaluations interleave */
-    const bool in = y > -1075.0;
-    const double yc = in ? y : 0.0;
+    const double yc = std::fmax(y, -1075.0);
     const double k = std::rint(yc * 64.0);
     const double x = (yc - k * 0.015625) * kLn2;
     const int ki = (int)k;
@@ -116,8 +123,7 @@ GGDMC_FN double exp2_neg(double y) {
     const double p =
         1.0 + x * (1.0 + x * (0.5 + x * (0.16666666666666666 + x * (0.041666666666666664 +
                                                                   x * 0.008333333333333333))));
-    const double r = std::ldexp(Ops::exp2_frac(ki & 63) * p, ki >> 6);
-    return in ? r : (y == y ? 0.0 : y);
+    return std::ldexp(Ops::exp2_frac(ki & 63) * p, ki >> 6);
 }
 
 /* 2^(j/64), j = 0..63, correctly rounded to FP64 (the table of exp2_neg). */
@@ -125,12 +131,13 @@ inline void fill_exp2_table(double *t) {
     for (int j = 0; j < 64; ++j) t[j] = (double)std::exp2((long double)j / 64.0L);
 }
 
-/* e = exp(-z^2/2), qa = Q(|z|) = upper normal tail beyond |z|. */
+/* e = exp(-z^2/2), qa = Q(|z|) = upper normal tail beyond |z| (NaN for a NaN
+ * z, through t; e is then 0). */
 template <class Ops>
 GGDMC_FN void mixed_tail(double z, double &e, double &qa) {
     e = exp2_neg<Ops>(-0.5 * z * z * kLog2e);
     const double a = std::fabs(z);
-    const double t = rcp64<Ops>(1.0 + kMillsK * a);
+    const double t = rcp64_ge1<Ops>(1.0 + kMillsK * a);
     /* sum_k c_k t^k, k = 0..11, by Horner's rule: each multiply-add reads one
      * coefficient (a scalar register), so no coefficient is first moved into
      * vector registers; the two trials and two tails a lane evaluates give the
