@@ -234,9 +234,9 @@ GGDMC_FN double div_rn(double x, double d, double y) {
     return std::isinf(x) ? q : c;
 }
 
-/* Winner: log(max(pdf, 1e-10)); NaN propagates. */
+/* Winner: max(pdf, 1e-10), the reference's floored density; NaN propagates. */
 template <class Ops>
-GGDMC_FN double mixed_winner_logpdf(const AccK &k, double rt, bool &ill) {
+GGDMC_FN double mixed_winner_pdf(const AccK &k, double rt, bool &ill) {
     /* branch-free: t <= 0 (or NaN) is evaluated at t = 1 and replaced */
     const double t = rt - k.t0;
     const bool pos = t > 0.0;
@@ -267,8 +267,15 @@ GGDMC_FN double mixed_winner_logpdf(const AccK &k, double rt, bool &ill) {
     const double pdf = num * k.invA * k.inv_den;
     /* both floored unless the pdf or its uncertainty reaches the floor */
     ill |= pos && err > kCondTol * std::fabs(num) && (std::fabs(num) + err) * k.invA * k.inv_den > 0.5 * kFloor;
-    const double lp = pdf < kFloor ? kLogFloor : mixed_log<Ops>(pdf);
-    return pos ? lp : (t == t ? kLogFloor : t);
+    const double pf = pdf < kFloor ? kFloor : pdf;
+    return pos ? pf : (t == t ? kFloor : t);
+}
+
+/* Winner: log(max(pdf, 1e-10)); NaN propagates. */
+template <class Ops>
+GGDMC_FN double mixed_winner_logpdf(const AccK &k, double rt, bool &ill) {
+    const double pf = mixed_winner_pdf<Ops>(k, rt, ill);
+    return pf == kFloor ? kLogFloor : mixed_log<Ops>(pf);
 }
 
 /* Loser: the reference's FP64 survivor 1 - max(F, 1e-10) (may be <= 0). */
@@ -299,13 +306,6 @@ GGDMC_FN double mixed_loser_survivor(const AccK &k, double rt, bool &ill) {
     return pos ? S0 : (t == t ? 1.0 - kFloor : t);
 }
 
-/* Accumulates one survivor factor into (log |prod|, sign, zero) state. */
-GGDMC_FN void fold_survivor(double S, double logS, double &acc, int &neg, bool &zero) {
-    if (S == 0.0) zero = true;
-    if (S < 0.0) neg ^= 1;
-    acc += logS;
-}
-
 /* Reference FP64 trial term (src formula above), for the plain path and for
  * the batched path's ill-conditioned trials. */
 template <class Ops>
@@ -324,26 +324,24 @@ GGDMC_FN double naive_trial_term(const AccK *kc, int n_acc, int w, double rt) {
     return Ops::log(dens <= 0.0 ? 2.220446049250313e-16 : dens);
 }
 
-/* Batched-path trial term, mixed evaluation; ill is set when any
- * accumulator's evaluation is ill-conditioned (the caller then takes
- * naive_trial_term instead). NaN propagates without the flag. */
+/* Batched-path trial term, mixed evaluation: the reference's density
+ * max(pdf_w, 1e-10) * prod_{j != w} (1 - max(F_j, 1e-10)), multiplied in FP64
+ * in accumulator order as the reference does, and one log of it (that of
+ * DBL_EPSILON for a density <= 0). A survivor is 0, or at least 2^-53 in
+ * magnitude, so the product of n_acc - 1 <= 18 of them and the floored pdf
+ * stays normal. ill is set when any accumulator's evaluation is
+ * ill-conditioned (the caller then takes naive_trial_term instead). NaN
+ * propagates without the flag. */
 template <class Ops>
 GGDMC_FN double mixed_trial_term_flagged(const AccK *kc, int n_acc, int w, double rt, bool &ill) {
-    const double lp = mixed_winner_logpdf<Ops>(kc[w], rt, ill);
-    double acc = 0.0, nanv = lp;
-    int neg = 0;
-    bool zero = false, nan = lp != lp;
+    double dens = mixed_winner_pdf<Ops>(kc[w], rt, ill);
     for (int l = 0; l + 1 < n_acc; ++l) {
         const int j = l + (l >= w);
-        const double S = mixed_loser_survivor<Ops>(kc[j], rt, ill);
-        const bool sn = S != S;
-        nan |= sn;
-        nanv = sn ? S : nanv;
-        const double aS = std::fabs(S);
-        fold_survivor(S, mixed_log<Ops>(aS > 0.0 ? aS : 1.0), acc, neg, zero);
+        dens *= mixed_loser_survivor<Ops>(kc[j], rt, ill);
     }
+    const bool nan = dens != dens;
     ill = ill && !nan;
-    return nan ? nanv : ((zero || neg) ? kLogDblEps : lp + acc);
+    return nan ? dens : (dens > 0.0 ? mixed_log<Ops>(dens) : kLogDblEps);
 }
 
 /* Two independent trials' terms (A and B) in one body: both winner terms,
@@ -354,28 +352,18 @@ template <class Ops>
 GGDMC_FN void mixed_trial_term_flagged2(const AccK *kA, int wA, double rtA, const AccK *kB, int wB,
                                         double rtB, int n_acc, double &outA, double &outB, bool &illA,
                                         bool &illB) {
-    const double lpA = mixed_winner_logpdf<Ops>(kA[wA], rtA, illA);
-    const double lpB = mixed_winner_logpdf<Ops>(kB[wB], rtB, illB);
-    double accA = 0.0, accB = 0.0, nanvA = lpA, nanvB = lpB;
-    int negA = 0, negB = 0;
-    bool zeroA = false, zeroB = false, nanA = lpA != lpA, nanB = lpB != lpB;
+    double dA = mixed_winner_pdf<Ops>(kA[wA], rtA, illA);
+    double dB = mixed_winner_pdf<Ops>(kB[wB], rtB, illB);
     for (int l = 0; l + 1 < n_acc; ++l) {
         const int jA = l + (l >= wA), jB = l + (l >= wB);
-        const double SA = mixed_loser_survivor<Ops>(kA[jA], rtA, illA);
-        const double SB = mixed_loser_survivor<Ops>(kB[jB], rtB, illB);
-        const bool snA = SA != SA, snB = SB != SB;
-        nanA |= snA;
-        nanB |= snB;
-        nanvA = snA ? SA : nanvA;
-        nanvB = snB ? SB : nanvB;
-        const double aSA = std::fabs(SA), aSB = std::fabs(SB);
-        fold_survivor(SA, mixed_log<Ops>(aSA > 0.0 ? aSA : 1.0), accA, negA, zeroA);
-        fold_survivor(SB, mixed_log<Ops>(aSB > 0.0 ? aSB : 1.0), accB, negB, zeroB);
+        dA *= mixed_loser_survivor<Ops>(kA[jA], rtA, illA);
+        dB *= mixed_loser_survivor<Ops>(kB[jB], rtB, illB);
     }
+    const bool nanA = dA != dA, nanB = dB != dB;
     illA = illA && !nanA;
     illB = illB && !nanB;
-    outA = nanA ? nanvA : ((zeroA || negA) ? kLogDblEps : lpA + accA);
-    outB = nanB ? nanvB : ((zeroB || negB) ? kLogDblEps : lpB + accB);
+    outA = nanA ? dA : (dA > 0.0 ? mixed_log<Ops>(dA) : kLogDblEps);
+    outB = nanB ? dB : (dB > 0.0 ? mixed_log<Ops>(dB) : kLogDblEps);
 }
 
 /* Batched-path trial term: mixed evaluation, or the reference formula when
