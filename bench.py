@@ -52,18 +52,18 @@ W_ALG = {2: 130.0, 4: 255.0}
 # SIMD of the kernel build), for the
 # decompositions the autotuner picks on these workloads: a figure is reported
 # only for the decomposition it was counted on.
-W_ISSUED = {('c3', 1, 0, 4, 3): 805.7, ('c3', 1, 0, 3, 3): 959.8, ('c3', 2, 0, 4, 3): 691.6,
-            ('c3', 2, 0, 3, 3): 763.2, ('c4', 4, 0, 3, 4): 589.9, ('c4', 4, 0, 3, 3): 584.8,
-            ('c4', 4, 0, 4, 3): 554.9, ('c5', 17, 0, 3, 4): 844.3, ('c5', 17, 0, 3, 3): 844.1,
-            ('c5', 17, 0, 4, 3): 844.2}
+W_ISSUED = {('c3', 1, 0, 4, 3): 767.4, ('c3', 1, 0, 3, 3): 959.8, ('c3', 2, 0, 4, 3): 632.1,
+            ('c3', 2, 0, 3, 3): 724.7, ('c4', 4, 0, 3, 4): 515.5, ('c4', 4, 0, 3, 3): 525.7,
+            ('c4', 4, 0, 4, 3): 486.5, ('c5', 17, 0, 3, 4): 678.0, ('c5', 17, 0, 3, 3): 679.4,
+            ('c5', 17, 0, 4, 3): 677.5}
 # HBM bytes per iteration of the subject sweep kernel at those decompositions:
 # rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB (factors of a 256 MiB copy
 # kernel, same files) per launch, divided by the 32 iterations of a launch.
-TRAFFIC = {('c3', 1, 0, 4, 3): 1.501e+06, ('c3', 1, 0, 3, 3): 1.255e+06,
-           ('c3', 2, 0, 4, 3): 1.270e+06, ('c3', 2, 0, 3, 3): 1.252e+06,
-           ('c4', 4, 0, 3, 4): 8.071e+06, ('c4', 4, 0, 3, 3): 5.485e+06,
-           ('c4', 4, 0, 4, 3): 5.506e+06, ('c5', 17, 0, 3, 4): 2.593e+07,
-           ('c5', 17, 0, 3, 3): 2.195e+07, ('c5', 17, 0, 4, 3): 2.211e+07}
+TRAFFIC = {('c3', 1, 0, 4, 3): 1.511e+06, ('c3', 1, 0, 3, 3): 1.238e+06,
+           ('c3', 2, 0, 4, 3): 1.258e+06, ('c3', 2, 0, 3, 3): 1.237e+06,
+           ('c4', 4, 0, 3, 4): 5.699e+06, ('c4', 4, 0, 3, 3): 5.475e+06,
+           ('c4', 4, 0, 4, 3): 5.493e+06, ('c5', 17, 0, 3, 4): 2.181e+07,
+           ('c5', 17, 0, 3, 3): 2.181e+07, ('c5', 17, 0, 4, 3): 2.195e+07}
 # Iterations a bench session holds at least, whatever --steps asks: the
 # library autotunes the work decomposition of sessions of 64 or more
 # iterations, so the measured configuration never depends on the step count.
