@@ -24,9 +24,10 @@
  *        (short polynomial), all FP64;
  *      * 1/x from v_rcp_f32 plus one FP64 Newton step;
  *      * Q(|z|) = exp(-z^2/2) * R(|z|), R the Mills-ratio factor
- *        Q(z)*exp(z^2/2), approximated as t*sum_k c_k t^k with t = 1/(1+0.275|z|)
- *        (coefficients fitted for this library by weighted least squares to
- *        0.5*erfcx(z/sqrt(2)) on [0, 14]; max relative error 8.2e-10);
+ *        Q(z)*exp(z^2/2), approximated as t*sum_k c_k t^k, k = 0..10, with
+ *        t = 1/(1+0.2625|z|) (coefficients fitted for this library to
+ *        0.5*erfcx(z/sqrt(2)) on [0, 14] by least squares reweighted toward
+ *        the minimax relative error; max relative error 2.9e-10);
  *      * the small tail of Phi is always the one computed, so
  *        Phi(z1) - Phi(z2) is taken as Q(z2) - Q(z1) when z2 >= 0; where
  *        the reference's own FP64 value RN(1 - Q(z1)) - RN(1 - Q(z2)) could
@@ -72,13 +73,13 @@ constexpr double kInvSqrt2Pi = 0.3989422804014327;
 constexpr double kLog2e = 1.4426950408889634;
 constexpr double kLn2 = 0.6931471805599453;
 constexpr double kSqrt1_2 = 0.7071067811865476;
-constexpr double kMillsK = 0.275;
-constexpr int kMillsDeg = 11;
-/* R(z) ~= t * sum_{k=0}^{11} c_k t^k, t = 1/(1 + 0.275 z), z >= 0 */
+constexpr double kMillsK = 0.2625;
+constexpr int kMillsDeg = 10;
+/* R(z) ~= t * sum_{k=0}^{10} c_k t^k, t = 1/(1 + 0.2625 z), z >= 0 */
 constexpr double kMills[kMillsDeg + 1] = {
-    0.10970522021435668,  0.10981321934884514, 0.10017863047558236, 0.09341028453443045,
-    0.022740656309337238, 0.1579382946903426,  -0.252144678841467,  0.39597478558978705,
-    -0.41305660985486853, 0.23823343654297754, -0.0717659160056362, 0.008972676989147887};
+    0.10471908204997292, 0.10480896722129296, 0.09649084604609494,  0.090014727337448,
+    0.032247173067295534, 0.13170626451118267, -0.1694328833813132, 0.2609796199857909,
+    -0.23411156866613111, 0.09871999173116763, -0.016142220044834274};
 
 /* Per (chain, cell, accumulator) constants, computed once per proposal. */
 struct AccK {
@@ -135,16 +136,16 @@ inline void fill_exp2_table(double *t) {
  * z, through t; e is then 0). */
 template <class Ops>
 GGDMC_FN void mixed_tail(double z, double &e, double &qa) {
-    e = exp2_neg<Ops>(-0.5 * z * z * kLog2e);
+    e = exp2_neg<Ops>((z * z) * (-0.5 * kLog2e));
     const double a = std::fabs(z);
     const double t = rcp64_ge1<Ops>(1.0 + kMillsK * a);
-    /* sum_k c_k t^k, k = 0..11, by Horner's rule: each multiply-add reads one
+    /* sum_k c_k t^k, k = 0..10, by Horner's rule: each multiply-add reads one
      * coefficient (a scalar register), so no coefficient is first moved into
      * vector registers; the two trials and two tails a lane evaluates give the
      * dependency chains to interleave */
-    double h = kMills[11];
+    double h = kMills[kMillsDeg];
 #pragma unroll
-    for (int j = 10; j >= 0; --j) h = kMills[j] + t * h;
+    for (int j = kMillsDeg - 1; j >= 0; --j) h = kMills[j] + t * h;
     qa = e * t * h;
 }
 
@@ -219,7 +220,7 @@ GGDMC_FN void naive_pdf_cdf(const AccK &k, double t, double &pdf, double &cdf) {
  * near 1, ulp 1.1e-16, scaled by the formula's coefficients), is flagged
  * ill-conditioned and the trial is evaluated with the reference's FP64
  * formula instead. */
-constexpr double kTailErr = 1e-9;   /* Mills-ratio fit (8.2e-10) */
+constexpr double kTailErr = 1e-9;   /* Mills-ratio fit (2.9e-10) and the FP32 seeds */
 constexpr double kExpErr = 1e-15;   /* exp(-z^2/2) in FP64 arithmetic */
 constexpr double kCondTol = 1e-5;
 constexpr double kUlp1 = 1.1102230246251565e-16;
