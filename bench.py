@@ -52,18 +52,18 @@ W_ALG = {2: 130.0, 4: 255.0}
 # SIMD of the kernel build), for the
 # decompositions the autotuner picks on these workloads: a figure is reported
 # only for the decomposition it was counted on.
-W_ISSUED = {('c3', 1, 0, 4, 3): 767.4, ('c3', 1, 0, 3, 3): 959.8, ('c3', 2, 0, 4, 3): 632.1,
-            ('c3', 2, 0, 3, 3): 724.7, ('c4', 4, 0, 3, 4): 515.5, ('c4', 4, 0, 3, 3): 525.7,
-            ('c4', 4, 0, 4, 3): 486.5, ('c5', 17, 0, 3, 4): 678.0, ('c5', 17, 0, 3, 3): 679.4,
-            ('c5', 17, 0, 4, 3): 677.5}
+W_ISSUED = {('c3', 1, 0, 4, 3): 735.6, ('c3', 1, 0, 3, 3): 889.1, ('c3', 2, 0, 4, 3): 607.4,
+            ('c3', 2, 0, 3, 3): 683.7, ('c4', 4, 0, 3, 4): 519.7, ('c4', 4, 0, 3, 3): 504.5,
+            ('c4', 4, 0, 4, 3): 473.8, ('c5', 17, 0, 3, 4): 664.3, ('c5', 17, 0, 3, 3): 661.6,
+            ('c5', 17, 0, 4, 3): 659.9}
 # HBM bytes per iteration of the subject sweep kernel at those decompositions:
 # rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB (factors of a 256 MiB copy
 # kernel, same files) per launch, divided by the 32 iterations of a launch.
-TRAFFIC = {('c3', 1, 0, 4, 3): 1.511e+06, ('c3', 1, 0, 3, 3): 1.238e+06,
-           ('c3', 2, 0, 4, 3): 1.258e+06, ('c3', 2, 0, 3, 3): 1.237e+06,
-           ('c4', 4, 0, 3, 4): 5.699e+06, ('c4', 4, 0, 3, 3): 5.475e+06,
-           ('c4', 4, 0, 4, 3): 5.493e+06, ('c5', 17, 0, 3, 4): 2.181e+07,
-           ('c5', 17, 0, 3, 3): 2.181e+07, ('c5', 17, 0, 4, 3): 2.195e+07}
+TRAFFIC = {('c3', 1, 0, 4, 3): 1.498e+06, ('c3', 1, 0, 3, 3): 1.238e+06,
+           ('c3', 2, 0, 4, 3): 1.250e+06, ('c3', 2, 0, 3, 3): 1.231e+06,
+           ('c4', 4, 0, 3, 4): 5.701e+06, ('c4', 4, 0, 3, 3): 5.456e+06,
+           ('c4', 4, 0, 4, 3): 5.483e+06, ('c5', 17, 0, 3, 4): 2.212e+07,
+           ('c5', 17, 0, 3, 3): 2.170e+07, ('c5', 17, 0, 4, 3): 2.186e+07}
 # Iterations a bench session holds at least, whatever --steps asks: the
 # library autotunes the work decomposition of sessions of 64 or more
 # iterations, so the measured configuration never depends on the step count.
