@@ -49,6 +49,8 @@
  *        1 - G = (Phi2 - Pc) + (chi*(Phi1 - Phi2) + zs*(phi1 - phi2)) / A + Pc
  *    (the first two terms from integrating the drift density over the start
  *    point: P(0 < drift < (b - a)/t) averaged over a ~ U(0, A)).
+ *    The trial's density is the reference's own FP64 product of the floored
+ *    winner pdf and the survivors, in accumulator order, logged once:
  *    log(x) = e*ln2 + log2(m)*ln2 with the FP32 log2 applied to the mantissa
  *    only, so its error is absolute ~1e-8 whatever the magnitude of log(x).
  */
