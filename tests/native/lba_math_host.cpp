@@ -55,6 +55,11 @@ void host_acc_eval(const double *pars, double rt, double *out4) { /* out4: 6 val
     out4[5] = ill_l;
 }
 
+// The batched path's normal tail: e[i] = exp(-z^2/2), q[i] = Q(|z[i]|).
+void host_mixed_tail(const double *z, int64_t n, double *e, double *q) {
+    for (int64_t i = 0; i < n; ++i) ggdmc::mixed_tail<HostOps>(z[i], e[i], q[i]);
+}
+
 // Per-trial log terms (mixed precision) for one subject; returns the sum.
 double host_mixed_terms(const int32_t *winner, const int32_t *par_src, const double *par_const,
                         int32_t n_cell, int32_t n_acc, const double *rt, const int64_t *cell_off,

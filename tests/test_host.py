@@ -157,6 +157,29 @@ def test_ill_conditioned_evaluations_fall_back_to_the_reference_formula():
     assert flagged < 0.05 * n
 
 
+def test_mixed_tail_accuracy():
+    # the batched path's normal tail Q(|z|) = exp(-z^2/2) R(|z|) against
+    # erfc: the error model takes every computed tail to 1e-9 relative
+    # (kTailErr), the Mills-ratio fit to 2.9e-10 on [0, 14]; exp(-z^2/2) to
+    # the rounding of its argument (z^2/2 in two roundings) plus 4e-16
+    from scipy.special import erfc
+    L = C.CDLL(_HOST)
+    d = C.POINTER(C.c_double)
+    L.host_mixed_tail.argtypes = [d, C.c_int64, d, d]
+    z = np.concatenate([np.linspace(-14, 14, 400001), np.geomspace(1e-12, 14, 100000), [0.0, -0.0]])
+    e, q = np.empty_like(z), np.empty_like(z)
+    L.host_mixed_tail(z.ctypes.data_as(d), len(z), e.ctypes.data_as(d), q.ctypes.data_as(d))
+    rq = np.abs(q / (0.5 * erfc(np.abs(z) * np.sqrt(0.5))) - 1)
+    re = np.abs(e / np.exp(-0.5 * z * z) - 1)
+    assert rq.max() < 4e-10, (rq.max(), z[rq.argmax()])
+    assert np.all(re <= 4e-16 + 1.2e-16 * z * z), (re.max(), z[re.argmax()])
+    # beyond the range of doubles the tail and exp are 0; NaN propagates through q
+    big = np.array([40.0, -1e10, np.inf, np.nan])
+    e, q = np.empty_like(big), np.empty_like(big)
+    L.host_mixed_tail(big.ctypes.data_as(d), len(big), e.ctypes.data_as(d), q.ctypes.data_as(d))
+    assert np.all(e[:3] == 0.0) and np.all(q[:2] < 1e-300) and np.isnan(q[3])
+
+
 def test_division_from_reciprocal_is_correctly_rounded():
     # the loser survivor's F = (1 - omG) / den divides through the correctly
     # rounded reciprocal with one correction step: bit for bit IEEE division
